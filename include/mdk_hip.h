@@ -1,0 +1,116 @@
+/*
+ * libmdk -- MI355X (gfx950) FFT docking rotation loop behind a C ABI.
+ *
+ * The entry points replace the GPU side of the reference's FFTProcess class
+ * (src/fft_process.h:45-196); each declaration cites the reference interface it
+ * stands in for.  All functions return 0 on success and a nonzero status otherwise;
+ * mdk_last_error() then describes the failure.  No exception and no exit() crosses
+ * this boundary: the host maps a nonzero status onto the reference's fatal path.
+ *
+ * Buffers passed in are host memory unless stated otherwise; they are copied, and the
+ * device memory belongs to the context.  A context is used by one host thread at a
+ * time (the reference binds OpenMP thread myid2 to device myid2,
+ * src/fft_process.cpp:527).
+ */
+#ifndef MDK_HIP_H
+#define MDK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Layout-identical to SortScore (src/fft_process.h:40-43):
+ * index = {rotation, tx, ty, tz}. */
+typedef struct {
+    float score;
+    int index[4];
+} mdk_sortscore;
+
+typedef struct mdk_ctx mdk_ctx;
+
+/* Library version (no reference counterpart). */
+int mdk_version(void);
+
+/* Description of the last failure in this thread (no reference counterpart; the
+ * reference prints and exits, e.g. src/fft_process.cpp:538-541). */
+const char *mdk_last_error(void);
+
+/* Host helper: ZXZ rotation coefficients (R*9, row-major r11..r33) for Euler triples
+ * theta (R*3), in float with cosf/sinf and the reference's expression order
+ * (Docking::ligand_rotationz, src/docking.cpp:376-384). */
+int mdk_rotation_coef(const float *theta, int R, float *rotcoef);
+
+/* Number of visible GPUs (replaces cudaGetDeviceCount in src/main.cpp:87-109). */
+int mdk_device_count(int *count);
+
+/* Allocate device state for one receptor-ligand pair: grids of nf^3 points, ligand
+ * voxel grids of ng^3, na_lig ligand atoms, t results per rotation (1..16) and
+ * rotation batches of up to max_batch.
+ * Replaces FFTProcess::alloc_array / alloc_fft (src/fft_process.cpp:29-209). */
+int mdk_create(int device, int nf, int ng, int na_lig, int t, int max_batch, mdk_ctx **out);
+
+/* Free everything (FFTProcess::fft_memory_free, src/fft_process.cpp:814-875). */
+int mdk_destroy(mdk_ctx *ctx);
+
+/* Receptor grids (ng^3 floats each, row-major x slowest): embedded in the nf^3 grid
+ * and transformed on the device (FFTProcess::receptor_fft, src/fft_process.cpp:212-303). */
+int mdk_set_receptor(mdk_ctx *ctx, const float *grid_r, const float *grid_i);
+
+/* Receptor spectrum given directly, SoA nf^3 floats each (the _FFT_rec_r/_FFT_rec_i
+ * arrays of src/fft_process.cpp:257-260). */
+int mdk_set_receptor_spectrum(mdk_ctx *ctx, const float *re, const float *im);
+
+/* Read the receptor spectrum back (SoA, nf^3 each). */
+int mdk_get_receptor_spectrum(mdk_ctx *ctx, float *re, float *im);
+
+/* Ligand atoms: xyz (na*3, PDB coordinates), center (3), radius, charge (na), grid
+ * coordinates (ng, src/docking.cpp:139-141), voxel pitch gw and core value (-b).
+ * Replaces FFTProcess::ligand_data_transfer_gpu (src/fft_process.cpp:769-807). */
+int mdk_set_ligand(mdk_ctx *ctx, const float *xyz, const float *center, const float *radius,
+                   const float *charge, const float *grid_coord, float gw, float core_delta);
+
+/* Dock B rotations given by host-computed coefficients rotcoef (B*9, row-major
+ * r11..r33 as src/docking.cpp:376-384 computes them in float) with rotation numbers
+ * rot_id (B); writes B*t results (the _Top entries of src/fft_process.cpp:640-643).
+ * Replaces FFTProcess::cuda_fft + ligand_voxelization_on_gpu
+ * (src/fft_process.cpp:502-765), batched. */
+int mdk_dock(mdk_ctx *ctx, const float *rotcoef, const int *rot_id, int B, mdk_sortscore *out);
+
+/* Device-resident rotation table (R*9 coefficients, rotation i has number i), so that
+ * mdk_dock_range runs without host traffic (the _Zangle table, src/parameter.h:87). */
+int mdk_set_rotations(mdk_ctx *ctx, const float *rotcoef, int R);
+
+/* Dock rotations [first, first+count) of the table, asynchronously, into the
+ * device-resident result array (R*t entries, the reference's _Top vector). */
+int mdk_dock_range(mdk_ctx *ctx, int first, int count);
+
+/* Wait for queued work. */
+int mdk_sync(mdk_ctx *ctx);
+
+/* Copy results of rotations [first, first+count) to the host (count*t entries). */
+int mdk_get_results(mdk_ctx *ctx, int first, int count, mdk_sortscore *out);
+
+/* Merge the results of rotations [0, nrot) on the device and return the best n, ordered
+ * by score descending, ties by rotation then rank ascending.  Replaces
+ * FFTProcess::sort_index (src/fft_process.cpp:469-479), whose tie order is unspecified. */
+int mdk_topn(mdk_ctx *ctx, int nrot, int n, mdk_sortscore *out);
+
+/* Time mdk_dock_range(first, count) `reps` times with HIP events on the context's
+ * stream; *ms = mean milliseconds per repetition. */
+int mdk_time_range(mdk_ctx *ctx, int first, int count, int reps, float *ms);
+
+/* Test hook: ligand voxel grids for B rotations (B*ng^3 floats each). */
+int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float *grid_i);
+
+/* Test hook: the full real score map (unnormalised, nf^3 floats, x slowest) of one
+ * rotation, i.e. Re IFFT(conj(R) * FFT(ligand)). */
+int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
+
+/* Test hook: `count` independent n-point complex FFTs (interleaved complex64), sign
+ * -1 forward / +1 inverse, unnormalised, on device `device`. */
+int mdk_fft1d(int device, int n, int count, const float *in, float *out, int sign);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

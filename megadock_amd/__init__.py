@@ -1,0 +1,181 @@
+"""megadock_amd: MI355X-native FFT docking rotation loop.
+
+Python face of ``libmdk.so`` (the HIP kernels behind the C ABI declared in
+``include/mdk_hip.h``).  Loading fails loudly when the library has not been built;
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmdk.so")
+DATA = os.path.join(HERE, "data")
+
+SORTSCORE_DTYPE = np.dtype([("score", np.float32), ("index", np.int32, (4,))])
+
+# every symbol declared in include/mdk_hip.h
+ABI_SYMBOLS = [
+    "mdk_version", "mdk_last_error", "mdk_rotation_coef", "mdk_device_count", "mdk_create", "mdk_destroy",
+    "mdk_set_receptor", "mdk_set_receptor_spectrum", "mdk_get_receptor_spectrum",
+    "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
+    "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_voxelize", "mdk_score_map",
+    "mdk_fft1d",
+]
+
+_lib = None
+
+
+class MdkError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmdk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MdkError("libmdk.so not built (run __graft_entry__.build() or make -C megadock_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        L.mdk_last_error.restype = C.c_char_p
+        L.mdk_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        for name in ABI_SYMBOLS:
+            getattr(L, name)  # raises AttributeError if an export is missing
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise MdkError("%s failed: %s" % (what, lib().mdk_last_error().decode()))
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().mdk_device_count(C.byref(n)), "mdk_device_count")
+    return n.value
+
+
+def rotation_coefs(angles):
+    """float32[R,9] ZXZ coefficients computed on the host by mdk_rotation_coef
+    (float cosf/sinf, src/docking.cpp:376-384)."""
+    a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
+    r = np.empty((len(a), 9), np.float32)
+    _check(lib().mdk_rotation_coef(_fp(a), C.c_int(len(a)), _fp(r)), "mdk_rotation_coef")
+    return r
+
+
+class Context:
+    """One receptor-ligand docking context on one GPU (wraps mdk_ctx)."""
+
+    def __init__(self, nf, ng, na, t=1, max_batch=16, device=0):
+        self.nf, self.ng, self.na, self.t, self.max_batch = nf, ng, na, t, max_batch
+        h = C.c_void_p()
+        _check(lib().mdk_create(device, nf, ng, na, t, max_batch, C.byref(h)), "mdk_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mdk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_receptor(self, grid_r, grid_i):
+        gr = np.ascontiguousarray(grid_r, np.float32)
+        gi = np.ascontiguousarray(grid_i, np.float32)
+        assert gr.size == self.ng ** 3 and gi.size == self.ng ** 3
+        _check(lib().mdk_set_receptor(self.h, _fp(gr), _fp(gi)), "mdk_set_receptor")
+
+    def set_receptor_spectrum(self, re, im):
+        re = np.ascontiguousarray(re, np.float32)
+        im = np.ascontiguousarray(im, np.float32)
+        assert re.size == self.nf ** 3 and im.size == self.nf ** 3
+        _check(lib().mdk_set_receptor_spectrum(self.h, _fp(re), _fp(im)), "mdk_set_receptor_spectrum")
+
+    def receptor_spectrum(self):
+        n3 = self.nf ** 3
+        re = np.zeros(n3, np.float32)
+        im = np.zeros(n3, np.float32)
+        _check(lib().mdk_get_receptor_spectrum(self.h, _fp(re), _fp(im)), "mdk_get_receptor_spectrum")
+        return re, im
+
+    def set_ligand(self, xyz, center, radius, charge, grid_coord, gw, delta=1.0):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        assert xyz.size == 3 * self.na
+        args = [np.ascontiguousarray(a, np.float32) for a in (center, radius, charge, grid_coord)]
+        _check(lib().mdk_set_ligand(self.h, _fp(xyz), *[_fp(a) for a in args], C.c_float(gw), C.c_float(delta)),
+               "mdk_set_ligand")
+
+    def dock(self, rotcoef, rot_id):
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        ids = np.ascontiguousarray(rot_id, np.int32)
+        B = len(rc)
+        out = np.zeros(B * self.t, SORTSCORE_DTYPE)
+        _check(lib().mdk_dock(self.h, _fp(rc), _ip(ids), C.c_int(B), C.c_void_p(out.ctypes.data)), "mdk_dock")
+        return out
+
+    def set_rotations(self, rotcoef):
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        self.nrot = len(rc)
+        _check(lib().mdk_set_rotations(self.h, _fp(rc), C.c_int(len(rc))), "mdk_set_rotations")
+
+    def dock_range(self, first, count):
+        _check(lib().mdk_dock_range(self.h, C.c_int(first), C.c_int(count)), "mdk_dock_range")
+
+    def sync(self):
+        _check(lib().mdk_sync(self.h), "mdk_sync")
+
+    def results(self, first, count):
+        out = np.zeros(count * self.t, SORTSCORE_DTYPE)
+        _check(lib().mdk_get_results(self.h, C.c_int(first), C.c_int(count), C.c_void_p(out.ctypes.data)),
+               "mdk_get_results")
+        return out
+
+    def topn(self, nrot, n):
+        out = np.zeros(n, SORTSCORE_DTYPE)
+        _check(lib().mdk_topn(self.h, C.c_int(nrot), C.c_int(n), C.c_void_p(out.ctypes.data)), "mdk_topn")
+        return out
+
+    def time_range(self, first, count, reps=1):
+        ms = C.c_float(0.0)
+        _check(lib().mdk_time_range(self.h, C.c_int(first), C.c_int(count), C.c_int(reps), C.byref(ms)),
+               "mdk_time_range")
+        return ms.value
+
+    def voxelize(self, rotcoef):
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        B = len(rc)
+        gr = np.zeros(B * self.ng ** 3, np.float32)
+        gi = np.zeros(B * self.ng ** 3, np.float32)
+        _check(lib().mdk_voxelize(self.h, _fp(rc), C.c_int(B), _fp(gr), _fp(gi)), "mdk_voxelize")
+        return gr.reshape(B, -1), gi.reshape(B, -1)
+
+    def score_map(self, rotcoef):
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(9)
+        out = np.zeros(self.nf ** 3, np.float32)
+        _check(lib().mdk_score_map(self.h, _fp(rc), _fp(out)), "mdk_score_map")
+        return out
+
+
+def fft1d(lines, sign=-1, device=0):
+    a = np.ascontiguousarray(lines, np.complex64)
+    n = a.shape[-1]
+    count = a.size // n
+    out = np.zeros_like(a)
+    _check(lib().mdk_fft1d(C.c_int(device), C.c_int(n), C.c_int(count), _fp(a.view(np.float32)),
+                           _fp(out.view(np.float32)), C.c_int(sign)), "mdk_fft1d")
+    return out

@@ -1,0 +1,835 @@
+// Batched FFT docking pipeline for MI355X (gfx950) and the libmdk C ABI.
+//
+// Per batch of B rotations (nf = 2 ng points per axis, nd = (nf - ng) / 2 offset of the
+// ligand inside the FFT grid, H = nf/2 + 1):
+//   voxelize      ligand grids lg[b][ng^3]                           (mdk_vox.hip)
+//   k_fwd_z       z-FFT of the ng^2 non-zero rows        -> a1[b][ng][ng][nf]
+//   k_fwd_y       y-FFT of the ng*nf non-zero columns     -> a2[b][ng][nf][nf]
+//   k_x_corr      x-FFT, conj(R)L, Hermitian fold, x-IFFT -> cc[b][nf][nf][Hp]
+//   k_inv_y       y-IFFT in place on the kz <= nf/2 half
+//   k_inv_z_top   z-IFFT of the Hermitian rows (real output) + per-row top-t
+//   k_reduce_top  per-rotation top-t, written as SortScore
+// The score is Re IFFT(conj(R) L) (src/fft_process.cpp:333-346, :386-388), which equals
+// IFFT of the Hermitian part Q(k) = (P(k) + conj(P(-k))) / 2 of P = conj(R) L; only the
+// kz <= nf/2 half of Q is formed, so the inverse transform moves half the data.
+// The zero padding of the ligand (7/8 of the grid) is never read: the forward passes
+// transform only the rows that hold ligand voxels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mdk_fft.h"
+#include "mdk_internal.h"
+#include "../../include/mdk_hip.h"
+
+#define MDK_MAXT 16
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+    g_err = msg;
+    return 1;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+constexpr int WZ = 16;   // lines per workgroup, z passes
+constexpr int WY = 16;   // lines per workgroup, y passes
+constexpr int WX = 8;    // kz columns per workgroup in the x pass (plus 8 partner columns)
+constexpr int NT = 256;  // threads per workgroup
+
+struct Cand {
+    float key;
+    int idx;
+};
+
+// candidate order: key descending, then linear index ascending
+__device__ __forceinline__ bool better(float ka, int ia, float kb, int ib) {
+    return ka > kb || (ka == kb && ia < ib);
+}
+
+// register top-t list with static indexing (t <= MDK_MAXT)
+struct TopList {
+    float k[MDK_MAXT];
+    int i[MDK_MAXT];
+    __device__ void init(int t, float k0) {
+#pragma unroll
+        for (int q = 0; q < MDK_MAXT; q++) {
+            k[q] = k0;
+            i[q] = 0;
+        }
+    }
+    __device__ void push(int t, float kc, int ic) {
+#pragma unroll
+        for (int q = 0; q < MDK_MAXT; q++) {
+            if (q < t && better(kc, ic, k[q], i[q])) {
+                const float tk = k[q];
+                const int ti = i[q];
+                k[q] = kc;
+                i[q] = ic;
+                kc = tk;
+                ic = ti;
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ void load_tw(float2 *tw_s, const float2 *tw_g, int ntw) {
+    for (int q = threadIdx.x; q < ntw; q += blockDim.x) tw_s[q] = tw_g[q];
+}
+
+// ---------------------------------------------------------------- forward z
+// lines: (b, x, y) with x, y < ng; row z < ng of lg sits at z + nd of the FFT line.
+__global__ void __launch_bounds__(NT) k_fwd_z(const float2 *__restrict__ lg, float2 *__restrict__ a1,
+                                              FftPlan p, const float2 *__restrict__ twg, int ng,
+                                              int nd, int nlines) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = WZ + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    load_tw(tw, twg, p.ntw);
+    const int line0 = blockIdx.x * WZ;
+    for (int q = threadIdx.x; q < WZ * nf; q += NT) {
+        const int l = q / nf, z = q % nf;
+        float2 v = make_float2(0.f, 0.f);
+        if (line0 + l < nlines && z >= nd && z < nd + ng) v = lg[(size_t)(line0 + l) * ng + (z - nd)];
+        A[z * LS + l] = v;
+    }
+    __syncthreads();
+    float2 *res = lds_fft<false>(A, Bf, tw, p, LS, threadIdx.x % WZ, threadIdx.x / WZ, NT / WZ, true);
+    for (int q = threadIdx.x; q < WZ * nf; q += NT) {
+        const int l = q / nf, kz = q % nf;
+        if (line0 + l < nlines) a1[(size_t)(line0 + l) * nf + kz] = res[kz * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- forward y
+// block (kz chunk, x, b); rows y < ng of a1 sit at y + nd.
+__global__ void __launch_bounds__(NT) k_fwd_y(const float2 *__restrict__ a1, float2 *__restrict__ a2,
+                                              FftPlan p, const float2 *__restrict__ twg, int ng,
+                                              int nd) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = WY + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    load_tw(tw, twg, p.ntw);
+    const int kz0 = blockIdx.x * WY, x = blockIdx.y, b = blockIdx.z;
+    const float2 *src = a1 + ((size_t)b * ng + x) * ng * nf;
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, y = q / WY;
+        float2 v = make_float2(0.f, 0.f);
+        if (kz0 + l < nf && y >= nd && y < nd + ng) v = src[(size_t)(y - nd) * nf + kz0 + l];
+        A[y * LS + l] = v;
+    }
+    __syncthreads();
+    float2 *res = lds_fft<false>(A, Bf, tw, p, LS, threadIdx.x % WY, threadIdx.x / WY, NT / WY, true);
+    float2 *dst = a2 + ((size_t)b * ng + x) * nf * nf;
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, ky = q / WY;
+        if (kz0 + l < nf) dst[(size_t)ky * nf + kz0 + l] = res[ky * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- x pass, receptor
+// plain forward x-FFT of the receptor: rec[kx][ky][kz] (full spectrum)
+__global__ void __launch_bounds__(NT) k_fwd_x_plain(const float2 *__restrict__ a2, float2 *__restrict__ rec,
+                                                    FftPlan p, const float2 *__restrict__ twg, int ng,
+                                                    int nd) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = WY + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    load_tw(tw, twg, p.ntw);
+    const int kz0 = blockIdx.x * WY, ky = blockIdx.y;
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, x = q / WY;
+        float2 v = make_float2(0.f, 0.f);
+        if (kz0 + l < nf && x >= nd && x < nd + ng) v = a2[((size_t)(x - nd) * nf + ky) * nf + kz0 + l];
+        A[x * LS + l] = v;
+    }
+    __syncthreads();
+    float2 *res = lds_fft<false>(A, Bf, tw, p, LS, threadIdx.x % WY, threadIdx.x / WY, NT / WY, true);
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, kx = q / WY;
+        if (kz0 + l < nf) rec[((size_t)kx * nf + ky) * nf + kz0 + l] = res[kx * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- x pass, docking
+// block (kz chunk c, ky, b).  Lines 0..WX-1: (ky, kz = c*WX + l); lines WX..2WX-1: the
+// partner columns (-ky, -kz).  Forward x-FFT of all 2*WX, then
+//   Q(k) = (conj(R(k)) L(k) + conj(conj(R(-k)) L(-k))) / 2
+// and an inverse x-FFT of the WX lines of Q.
+__global__ void __launch_bounds__(NT) k_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec,
+                                               float2 *__restrict__ cc, FftPlan p,
+                                               const float2 *__restrict__ twg, int ng, int nd, int H,
+                                               int Hp) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = 2 * WX + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    load_tw(tw, twg, p.ntw);
+    const int c = blockIdx.x, ky = blockIdx.y, b = blockIdx.z;
+    const int kyp = (nf - ky) % nf;
+    const float2 *src = a2 + (size_t)b * ng * nf * nf;
+    for (int q = threadIdx.x; q < 2 * WX * nf; q += NT) {
+        const int l = q % (2 * WX), x = q / (2 * WX);
+        const int kz = c * WX + (l % WX);
+        float2 v = make_float2(0.f, 0.f);
+        if (kz < H && x >= nd && x < nd + ng) {
+            if (l < WX) v = src[((size_t)(x - nd) * nf + ky) * nf + kz];
+            else v = src[((size_t)(x - nd) * nf + kyp) * nf + (nf - kz) % nf];
+        }
+        A[x * LS + l] = v;
+    }
+    __syncthreads();
+    float2 *L = lds_fft<false>(A, Bf, tw, p, LS, threadIdx.x % (2 * WX), threadIdx.x / (2 * WX),
+                               NT / (2 * WX), true);
+    float2 *Qb = (L == A) ? Bf : A;
+    for (int q = threadIdx.x; q < WX * nf; q += NT) {
+        const int l = q % WX, kx = q / WX;
+        const int kz = c * WX + l;
+        float2 Q = make_float2(0.f, 0.f);
+        if (kz < H) {
+            const int kxp = (nf - kx) % nf, kzp = (nf - kz) % nf;
+            const float2 r1 = rec[((size_t)kx * nf + ky) * nf + kz];
+            const float2 r2 = rec[((size_t)kxp * nf + kyp) * nf + kzp];
+            const float2 l1 = L[kx * LS + l], l2 = L[kxp * LS + WX + l];
+            // conj(r1) * l1 and conj(r2) * l2
+            const float2 p1 = make_float2(r1.x * l1.x + r1.y * l1.y, r1.x * l1.y - r1.y * l1.x);
+            const float2 p2 = make_float2(r2.x * l2.x + r2.y * l2.y, r2.x * l2.y - r2.y * l2.x);
+            Q = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
+        }
+        Qb[kx * LS + l] = Q;
+    }
+    __syncthreads();
+    float2 *other = (Qb == A) ? Bf : A;
+    float2 *res = lds_fft<true>(Qb, other, tw, p, LS, threadIdx.x % WX, threadIdx.x / WX, NT / WX, true);
+    float2 *dst = cc + (size_t)b * nf * nf * Hp;
+    for (int q = threadIdx.x; q < WX * nf; q += NT) {
+        const int l = q % WX, x = q / WX;
+        const int kz = c * WX + l;
+        if (kz < Hp) dst[((size_t)x * nf + ky) * Hp + kz] = res[x * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- inverse y (in place)
+__global__ void __launch_bounds__(NT) k_inv_y(float2 *__restrict__ cc, FftPlan p,
+                                              const float2 *__restrict__ twg, int Hp) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = WY + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    load_tw(tw, twg, p.ntw);
+    const int kz0 = blockIdx.x * WY, x = blockIdx.y, b = blockIdx.z;
+    float2 *base = cc + ((size_t)b * nf + x) * nf * Hp;
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, ky = q / WY;
+        A[ky * LS + l] = base[(size_t)ky * Hp + kz0 + l];
+    }
+    __syncthreads();
+    float2 *res = lds_fft<true>(A, Bf, tw, p, LS, threadIdx.x % WY, threadIdx.x / WY, NT / WY, true);
+    for (int q = threadIdx.x; q < WY * nf; q += NT) {
+        const int l = q % WY, y = q / WY;
+        base[(size_t)y * Hp + kz0 + l] = res[y * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- inverse z + top-t
+// rows r = (b*nf + x)*nf + y; the row's spectrum is Hermitian, only kz <= nf/2 stored.
+// Writes either the real score row (smap) or the row's top-t candidates.
+__global__ void __launch_bounds__(NT) k_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand,
+                                                  float *__restrict__ smap, FftPlan p,
+                                                  const float2 *__restrict__ twg, int H, int Hp,
+                                                  int nrows, int t, float nf3f) {
+    extern __shared__ float2 sm[];
+    const int nf = p.n, LS = WZ + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + nf * LS;
+    Cand *lists = reinterpret_cast<Cand *>(Bf + nf * LS);  // [WZ][NT/WZ][t]
+    load_tw(tw, twg, p.ntw);
+    const int row0 = blockIdx.x * WZ;
+    for (int q = threadIdx.x; q < WZ * H; q += NT) {
+        const int l = q / H, kz = q % H;
+        float2 v = make_float2(0.f, 0.f);
+        if (row0 + l < nrows) v = cc[(size_t)(row0 + l) * Hp + kz];
+        A[kz * LS + l] = v;
+        if (kz >= 1 && nf - kz > kz) A[(nf - kz) * LS + l] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    float2 *res = lds_fft<true>(A, Bf, tw, p, LS, threadIdx.x % WZ, threadIdx.x / WZ, NT / WZ, true);
+    const int l = threadIdx.x % WZ, jt = threadIdx.x / WZ, tpl = NT / WZ;
+    const int r = row0 + l;
+    if (smap) {
+        if (r < nrows)
+            for (int z = jt; z < nf; z += tpl) smap[(size_t)r * nf + z] = res[z * LS + l].x;
+        return;
+    }
+    // per-thread top-t over its elements; key = value (t == 1) or value / nf^3 (t > 1)
+    TopList tl;
+    tl.init(t, -INFINITY);
+    if (r < nrows) {
+        const int rr = r % (nf * nf);  // row within the rotation: x*nf + y
+        for (int z = jt; z < nf; z += tpl) {
+            const float v = res[z * LS + l].x;
+            const float key = (t == 1) ? v : v / nf3f;
+            tl.push(t, key, rr * nf + z);
+        }
+    }
+    Cand *my = lists + ((size_t)l * tpl + jt) * t;
+    for (int q = 0; q < t; q++) {
+#pragma unroll
+        for (int s = 0; s < MDK_MAXT; s++)
+            if (s == q) my[q] = Cand{tl.k[s], tl.i[s]};
+    }
+    __syncthreads();
+    if (jt == 0 && r < nrows) {
+        TopList m;
+        m.init(t, -INFINITY);
+        const Cand *all = lists + (size_t)l * tpl * t;
+        for (int q = 0; q < tpl * t; q++) m.push(t, all[q].key, all[q].idx);
+        for (int q = 0; q < t; q++) {
+#pragma unroll
+            for (int s = 0; s < MDK_MAXT; s++)
+                if (s == q) cand[(size_t)r * t + q] = Cand{m.k[s], m.i[s]};
+        }
+    }
+}
+
+// ---------------------------------------------------------------- per-rotation top-t
+// t == 1 starts from (0.0, index 0) and t > 1 from (-99999, index 0), matching the
+// reference's initial values and strict comparisons (src/fft_process.cpp:407-446).
+__global__ void __launch_bounds__(NT) k_reduce_top(const Cand *__restrict__ cand, mdk_sortscore *__restrict__ out,
+                                                   const int *__restrict__ rot_id, int rows_per_rot,
+                                                   int t, int nf, float nf3f) {
+    __shared__ Cand lists[NT * MDK_MAXT];
+    const int b = blockIdx.x;
+    const Cand *src = cand + (size_t)b * rows_per_rot * t;
+    const float k0 = (t == 1) ? 0.0f : -99999.0f;
+    TopList tl;
+    tl.init(t, k0);
+    const int n = rows_per_rot * t;
+    for (int q = threadIdx.x; q < n; q += NT) tl.push(t, src[q].key, src[q].idx);
+    for (int q = 0; q < t; q++) {
+#pragma unroll
+        for (int s = 0; s < MDK_MAXT; s++)
+            if (s == q) lists[threadIdx.x * t + q] = Cand{tl.k[s], tl.i[s]};
+    }
+    __syncthreads();
+    // tree merge of the NT lists (every thread reaches every barrier)
+    for (int half = NT / 2; half >= 1; half >>= 1) {
+        TopList m;
+        m.init(t, k0);
+        if (threadIdx.x < half) {
+            for (int q = 0; q < t; q++) {
+                const Cand a = lists[threadIdx.x * t + q], c = lists[(threadIdx.x + half) * t + q];
+                m.push(t, a.key, a.idx);
+                m.push(t, c.key, c.idx);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < half) {
+            for (int q = 0; q < t; q++) {
+#pragma unroll
+                for (int s = 0; s < MDK_MAXT; s++)
+                    if (s == q) lists[threadIdx.x * t + q] = Cand{m.k[s], m.i[s]};
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < t) {
+        const Cand c = lists[threadIdx.x];
+        mdk_sortscore s;
+        s.score = (t == 1) ? c.key / nf3f : c.key;
+        const int nf2 = nf * nf;
+        s.index[0] = rot_id[b];
+        s.index[1] = c.idx / nf2;
+        s.index[2] = (c.idx / nf) % nf;
+        s.index[3] = c.idx % nf;
+        out[(size_t)b * t + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_iota(int *p, int first, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = first + i;
+}
+
+// ---------------------------------------------------------------- 1D test kernel
+__global__ void __launch_bounds__(NT) k_lines(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                              FftPlan p, const float2 *__restrict__ twg, int nlines,
+                                              int inv) {
+    extern __shared__ float2 sm[];
+    const int n = p.n, LS = WZ + 1;
+    float2 *tw = sm, *A = sm + p.ntw, *Bf = A + n * LS;
+    load_tw(tw, twg, p.ntw);
+    const int line0 = blockIdx.x * WZ;
+    for (int q = threadIdx.x; q < WZ * n; q += NT) {
+        const int l = q / n, e = q % n;
+        A[e * LS + l] = (line0 + l < nlines) ? in[(size_t)(line0 + l) * n + e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 *res = inv ? lds_fft<true>(A, Bf, tw, p, LS, threadIdx.x % WZ, threadIdx.x / WZ, NT / WZ, true)
+                      : lds_fft<false>(A, Bf, tw, p, LS, threadIdx.x % WZ, threadIdx.x / WZ, NT / WZ, true);
+    for (int q = threadIdx.x; q < WZ * n; q += NT) {
+        const int l = q / n, e = q % n;
+        if (line0 + l < nlines) out[(size_t)(line0 + l) * n + e] = res[e * LS + l];
+    }
+}
+
+// ---------------------------------------------------------------- host-side plan
+// dynamic LDS above the default 64 KiB window must be requested per kernel
+template <typename K> hipError_t allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+bool make_plan(int n, FftPlan &p, std::vector<float2> &tw) {
+    static const int pref[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    int m = n, ns = 0;
+    int fac[MDK_MAX_STAGES];
+    // radix 8 first, then 4, a trailing 2 if needed, then odd primes
+    for (int a = 0; a < 8 && m > 1; a++)
+        while (m % pref[a] == 0) {
+            if (ns >= MDK_MAX_STAGES) return false;
+            fac[ns++] = pref[a];
+            m /= pref[a];
+        }
+    if (m != 1) return false;
+    p.n = n;
+    p.nst = ns;
+    tw.clear();
+    int Ns = 1;
+    for (int s = 0; s < ns; s++) {
+        const int R = fac[s];
+        p.radix[s] = R;
+        p.twoff[s] = (int)tw.size();
+        for (int r = 1; r < R; r++)
+            for (int k = 0; k < Ns; k++) {
+                const double a = -2.0 * M_PI * (double)r * k / ((double)Ns * R);
+                tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+            }
+        Ns *= R;
+    }
+    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    p.ntw = (int)tw.size();
+    return true;
+}
+
+}  // namespace
+
+// ==================================================================== context
+struct mdk_ctx {
+    int device = 0, nf = 0, ng = 0, na = 0, t = 1, max_batch = 1;
+    int nd = 0, H = 0, Hp = 0;
+    hipStream_t stream = nullptr;
+    FftPlan plan{};
+    float2 *d_tw = nullptr;
+    float2 *d_rec = nullptr;
+    bool have_rec = false, have_lig = false;
+    // ligand
+    float *d_xyz = nullptr, *d_rc2 = nullptr, *d_rs2 = nullptr, *d_charge = nullptr, *d_gc = nullptr;
+    LigandDev lig{};
+    // rotations
+    float *d_rot = nullptr;
+    int nrot = 0;
+    int *d_rotid = nullptr;  // iota table, rotation i has number i
+    float *d_brot = nullptr; // staging for mdk_dock (max_batch * 9)
+    int *d_bid = nullptr;    // staging ids
+    // batch buffers
+    uint32_t *d_bits = nullptr;
+    int *d_vidx = nullptr;
+    float2 *d_lg = nullptr, *d_a1 = nullptr, *d_a2 = nullptr, *d_cc = nullptr;
+    Cand *d_cand = nullptr;
+    mdk_sortscore *d_bout = nullptr;   // max_batch * t
+    mdk_sortscore *d_results = nullptr; // nrot * t
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+size_t lds_bytes(const mdk_ctx *c, int lines) {
+    return sizeof(float2) * ((size_t)c->plan.ntw + 2 * (size_t)c->nf * (lines + 1));
+}
+
+// Run the forward ligand passes + correlation + inverse passes for B rotations whose
+// voxel grids are already in d_lg; output top-t candidates (or the score map).
+int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev) {
+    const int ng = c->ng, nf = c->nf, nd = c->nd;
+    hipStream_t s = c->stream;
+    const int lines_z = B * ng * ng;
+    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
+    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, B), NT, lds_bytes(c, WY), s>>>(c->d_a1, c->d_a2, c->plan, c->d_tw, ng, nd);
+    k_x_corr<<<dim3(c->Hp / WX, nf, B), NT, lds_bytes(c, 2 * WX), s>>>(c->d_a2, c->d_rec, c->d_cc, c->plan, c->d_tw, ng, nd, c->H, c->Hp);
+    k_inv_y<<<dim3(c->Hp / WY, nf, B), NT, lds_bytes(c, WY), s>>>(c->d_cc, c->plan, c->d_tw, c->Hp);
+    const int rows = B * nf * nf;
+    const size_t lds_z = lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * c->t;
+    k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(c->d_cc, c->d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, c->t,
+                                                       (float)((size_t)nf * nf * nf));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev) {
+    HIPCHK(launch_voxelize(c->lig, rot_dev, B, c->d_bits, c->d_vidx, c->d_lg, c->stream));
+    if (run_fft_pipeline(c, B, nullptr)) return 1;
+    const int nf = c->nf;
+    k_reduce_top<<<B, NT, 0, c->stream>>>(c->d_cand, out_dev, id_dev, nf * nf, c->t, nf, (float)((size_t)nf * nf * nf));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename T> int dmalloc(T **p, size_t n) {
+    HIPCHK(hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+    return 0;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+int mdk_version(void) { return 1; }
+
+const char *mdk_last_error(void) { return g_err.c_str(); }
+
+int mdk_rotation_coef(const float *theta, int R, float *rotcoef) {
+#pragma clang fp contract(off)
+    if (!theta || !rotcoef || R < 0) return fail("mdk_rotation_coef: invalid argument");
+    for (int q = 0; q < R; q++) {
+        // src/docking.cpp:376-384: float cos/sin (std::cos(float) = cosf), float products
+        const float t0 = theta[3 * q], t1 = theta[3 * q + 1], t2 = theta[3 * q + 2];
+        float *r = rotcoef + 9 * (size_t)q;
+        r[0] = std::cos(t0) * std::cos(t2) - std::sin(t0) * std::cos(t1) * std::sin(t2);
+        r[3] = std::sin(t0) * std::cos(t2) + std::cos(t0) * std::cos(t1) * std::sin(t2);
+        r[6] = std::sin(t1) * std::sin(t2);
+        r[1] = -std::cos(t0) * std::sin(t2) - std::sin(t0) * std::cos(t1) * std::cos(t2);
+        r[4] = -std::sin(t0) * std::sin(t2) + std::cos(t0) * std::cos(t1) * std::cos(t2);
+        r[7] = std::sin(t1) * std::cos(t2);
+        r[2] = std::sin(t0) * std::sin(t1);
+        r[5] = -std::cos(t0) * std::sin(t1);
+        r[8] = std::cos(t1);
+    }
+    return 0;
+}
+
+int mdk_device_count(int *count) {
+    HIPCHK(hipGetDeviceCount(count));
+    return 0;
+}
+
+int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx **out) {
+    if (!out) return fail("mdk_create: null output pointer");
+    *out = nullptr;
+    if (nf < 4 || ng < 2 || ng > nf || na < 1 || t < 1 || t > MDK_MAXT || max_batch < 1)
+        return fail("mdk_create: invalid sizes (need 4 <= nf, 2 <= ng <= nf, na >= 1, 1 <= t <= 16, max_batch >= 1)");
+    if (nf != 2 * ng) return fail("mdk_create: nf must be 2*ng (src/docking.cpp:23-25)");
+    mdk_ctx *c = new mdk_ctx();
+    c->device = device;
+    c->nf = nf;
+    c->ng = ng;
+    c->na = na;
+    c->t = t;
+    c->max_batch = max_batch;
+    c->nd = (nf - ng) / 2;
+    c->H = nf / 2 + 1;
+    c->Hp = ((c->H + WY - 1) / WY) * WY;
+    std::vector<float2> tw;
+    if (!make_plan(nf, c->plan, tw)) {
+        delete c;
+        return fail("mdk_create: nf has a prime factor above 13");
+    }
+    auto bail = [&](int rc) {
+        mdk_destroy(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) { delete c; return fail("mdk_create: hipSetDevice failed"); }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail("mdk_create: stream creation failed");
+    }
+    const size_t nf3 = (size_t)nf * nf * nf, ng3 = (size_t)ng * ng * ng, B = max_batch;
+    if (dmalloc(&c->d_tw, tw.size()) || dmalloc(&c->d_rec, nf3) || dmalloc(&c->d_xyz, 3 * (size_t)na) ||
+        dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
+        dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) ||
+        dmalloc(&c->d_bits, 2 * B * mask_words(ng)) || dmalloc(&c->d_vidx, B * na) ||
+        dmalloc(&c->d_lg, B * ng3) || dmalloc(&c->d_a1, B * ng * ng * nf) ||
+        dmalloc(&c->d_a2, B * ng * nf * nf) || dmalloc(&c->d_cc, B * nf * nf * c->Hp) ||
+        dmalloc(&c->d_cand, B * nf * nf * t) || dmalloc(&c->d_bout, B * t))
+        return bail(1);
+    if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail("mdk_create: twiddle upload failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail("mdk_create: event creation failed"));
+    // the largest kernel LDS request must fit
+    const size_t lds_max = std::max(lds_bytes(c, 2 * WX), lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * t);
+    if (lds_max > 160 * 1024) return bail(fail("mdk_create: nf too large for the LDS-resident passes"));
+    if (allow_lds(k_fwd_z, lds_bytes(c, WZ)) != hipSuccess || allow_lds(k_fwd_y, lds_bytes(c, WY)) != hipSuccess ||
+        allow_lds(k_fwd_x_plain, lds_bytes(c, WY)) != hipSuccess ||
+        allow_lds(k_x_corr, lds_bytes(c, 2 * WX)) != hipSuccess || allow_lds(k_inv_y, lds_bytes(c, WY)) != hipSuccess ||
+        allow_lds(k_inv_z_top, lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * t) != hipSuccess)
+        return bail(fail("mdk_create: cannot raise the LDS limit"));
+    *out = c;
+    return 0;
+}
+
+int mdk_destroy(mdk_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_tw, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
+                    c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
+                    c->d_cc, c->d_cand, c->d_bout, c->d_results};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
+    if (!c || !grid_r || !grid_i) return fail("mdk_set_receptor: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const int ng = c->ng, nf = c->nf, nd = c->nd;
+    const size_t ng3 = (size_t)ng * ng * ng;
+    std::vector<float2> h(ng3);
+    for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
+    HIPCHK(hipMemcpyAsync(c->d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    const int lines_z = ng * ng;
+    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), c->stream>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
+    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->d_a1, c->d_a2, c->plan, c->d_tw, ng, nd);
+    k_fwd_x_plain<<<dim3((nf + WY - 1) / WY, nf), NT, lds_bytes(c, WY), c->stream>>>(c->d_a2, c->d_rec, c->plan, c->d_tw, ng, nd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_rec = true;
+    return 0;
+}
+
+int mdk_set_receptor_spectrum(mdk_ctx *c, const float *re, const float *im) {
+    if (!c || !re || !im) return fail("mdk_set_receptor_spectrum: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nf3 = (size_t)c->nf * c->nf * c->nf;
+    std::vector<float2> h(nf3);
+    for (size_t i = 0; i < nf3; i++) h[i] = make_float2(re[i], im[i]);
+    HIPCHK(hipMemcpy(c->d_rec, h.data(), sizeof(float2) * nf3, hipMemcpyHostToDevice));
+    c->have_rec = true;
+    return 0;
+}
+
+int mdk_get_receptor_spectrum(mdk_ctx *c, float *re, float *im) {
+    if (!c || !re || !im) return fail("mdk_get_receptor_spectrum: null argument");
+    if (!c->have_rec) return fail("mdk_get_receptor_spectrum: no receptor set");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nf3 = (size_t)c->nf * c->nf * c->nf;
+    std::vector<float2> h(nf3);
+    HIPCHK(hipMemcpy(h.data(), c->d_rec, sizeof(float2) * nf3, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nf3; i++) {
+        re[i] = h[i].x;
+        im[i] = h[i].y;
+    }
+    return 0;
+}
+
+int mdk_set_ligand(mdk_ctx *c, const float *xyz, const float *center, const float *radius,
+                   const float *charge, const float *grid_coord, float gw, float delta) {
+    if (!c || !xyz || !center || !radius || !charge || !grid_coord) return fail("mdk_set_ligand: null argument");
+    if (delta == 0.0f) return fail("mdk_set_ligand: core value 0 is not supported (the reference's last shape sweep is order-dependent then)");
+    if (!(gw > 0.0f)) return fail("mdk_set_ligand: grid width must be positive");
+    HIPCHK(hipSetDevice(c->device));
+    const int na = c->na;
+    std::vector<float> rc2(na), rs2(na);
+    for (int i = 0; i < na; i++) {  // src/fft_process.cpp:784-787 / src/ligand.cpp:254-257
+        rc2[i] = radius[i] * radius[i] * 1.5f;
+        rs2[i] = radius[i] * radius[i] * 1.0f;
+    }
+    HIPCHK(hipMemcpy(c->d_xyz, xyz, sizeof(float) * 3 * na, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rc2, rc2.data(), sizeof(float) * na, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rs2, rs2.data(), sizeof(float) * na, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_charge, charge, sizeof(float) * na, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_gc, grid_coord, sizeof(float) * c->ng, hipMemcpyHostToDevice));
+    LigandDev &L = c->lig;
+    L.na = na;
+    L.ng = c->ng;
+    L.gw = gw;
+    L.delta = delta;
+    L.pad = (float)c->ng * gw / 2;  // src/ligand.cpp:210
+    L.cx = center[0];
+    L.cy = center[1];
+    L.cz = center[2];
+    L.xyz = c->d_xyz;
+    L.rc2 = c->d_rc2;
+    L.rs2 = c->d_rs2;
+    L.charge = c->d_charge;
+    L.gc = c->d_gc;
+    c->have_lig = true;
+    return 0;
+}
+
+int mdk_dock(mdk_ctx *c, const float *rotcoef, const int *rot_id, int B, mdk_sortscore *out) {
+    if (!c || !rotcoef || !rot_id || !out) return fail("mdk_dock: null argument");
+    if (!c->have_rec || !c->have_lig) return fail("mdk_dock: receptor and ligand must be set first");
+    if (B < 1 || B > c->max_batch) return fail("mdk_dock: batch size out of range");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bid, rot_id, sizeof(int) * B, hipMemcpyHostToDevice, c->stream));
+    if (dock_batch(c, c->d_brot, c->d_bid, B, c->d_bout)) return 1;
+    HIPCHK(hipMemcpyAsync(out, c->d_bout, sizeof(mdk_sortscore) * B * c->t, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_set_rotations(mdk_ctx *c, const float *rotcoef, int R) {
+    if (!c || !rotcoef || R < 1) return fail("mdk_set_rotations: invalid argument");
+    HIPCHK(hipSetDevice(c->device));
+    if (c->d_rot) (void)hipFree(c->d_rot);
+    if (c->d_rotid) (void)hipFree(c->d_rotid);
+    if (c->d_results) (void)hipFree(c->d_results);
+    c->d_rot = nullptr;
+    c->d_rotid = nullptr;
+    c->d_results = nullptr;
+    if (dmalloc(&c->d_rot, 9 * (size_t)R) || dmalloc(&c->d_rotid, R) || dmalloc(&c->d_results, (size_t)R * c->t)) return 1;
+    HIPCHK(hipMemcpy(c->d_rot, rotcoef, sizeof(float) * 9 * R, hipMemcpyHostToDevice));
+    k_iota<<<(R + 255) / 256, 256, 0, c->stream>>>(c->d_rotid, 0, R);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nrot = R;
+    return 0;
+}
+
+int mdk_dock_range(mdk_ctx *c, int first, int count) {
+    if (!c) return fail("mdk_dock_range: null context");
+    if (!c->have_rec || !c->have_lig || !c->d_rot) return fail("mdk_dock_range: receptor, ligand and rotations must be set first");
+    if (first < 0 || count < 0 || first + count > c->nrot) return fail("mdk_dock_range: range outside the rotation table");
+    HIPCHK(hipSetDevice(c->device));
+    for (int b0 = first; b0 < first + count; b0 += c->max_batch) {
+        const int B = std::min(c->max_batch, first + count - b0);
+        if (dock_batch(c, c->d_rot + 9 * (size_t)b0, c->d_rotid + b0, B, c->d_results + (size_t)b0 * c->t)) return 1;
+    }
+    return 0;
+}
+
+int mdk_sync(mdk_ctx *c) {
+    if (!c) return fail("mdk_sync: null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_get_results(mdk_ctx *c, int first, int count, mdk_sortscore *out) {
+    if (!c || !out || !c->d_results) return fail("mdk_get_results: nothing to read");
+    if (first < 0 || count < 0 || first + count > c->nrot) return fail("mdk_get_results: range outside the rotation table");
+    HIPCHK(hipMemcpyAsync(out, c->d_results + (size_t)first * c->t, sizeof(mdk_sortscore) * count * c->t,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
+    if (!c || !out || !c->d_results) return fail("mdk_topn: nothing to merge");
+    if (nrot < 1 || nrot > c->nrot || n < 1) return fail("mdk_topn: invalid count");
+    // merge on the host for now: ordering key (score desc, rotation asc, rank asc)
+    const size_t m = (size_t)nrot * c->t;
+    std::vector<mdk_sortscore> all(m);
+    HIPCHK(hipMemcpyAsync(all.data(), c->d_results, sizeof(mdk_sortscore) * m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<size_t> order(m);
+    for (size_t i = 0; i < m; i++) order[i] = i;
+    const size_t keep = std::min((size_t)n, m);
+    std::partial_sort(order.begin(), order.begin() + keep, order.end(), [&](size_t a, size_t b) {
+        if (all[a].score != all[b].score) return all[a].score > all[b].score;
+        return a < b;
+    });
+    for (size_t i = 0; i < keep; i++) out[i] = all[order[i]];
+    return 0;
+}
+
+int mdk_time_range(mdk_ctx *c, int first, int count, int reps, float *ms) {
+    if (!c || !ms || reps < 1) return fail("mdk_time_range: invalid argument");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; r++)
+        if (mdk_dock_range(c, first, count)) return 1;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t / reps;
+    return 0;
+}
+
+int mdk_voxelize(mdk_ctx *c, const float *rotcoef, int B, float *grid_r, float *grid_i) {
+    if (!c || !rotcoef || !grid_r || !grid_i) return fail("mdk_voxelize: null argument");
+    if (!c->have_lig) return fail("mdk_voxelize: ligand not set");
+    if (B < 1 || B > c->max_batch) return fail("mdk_voxelize: batch size out of range");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, c->d_bits, c->d_vidx, c->d_lg, c->stream));
+    const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
+    std::vector<float2> h(B * ng3);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_lg, sizeof(float2) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < h.size(); i++) {
+        grid_r[i] = h[i].x;
+        grid_i[i] = h[i].y;
+    }
+    return 0;
+}
+
+int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
+    if (!c || !rotcoef || !out) return fail("mdk_score_map: null argument");
+    if (!c->have_rec || !c->have_lig) return fail("mdk_score_map: receptor and ligand must be set first");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nf3 = (size_t)c->nf * c->nf * c->nf;
+    float *d_map = nullptr;
+    HIPCHK(hipMalloc(&d_map, sizeof(float) * nf3));
+    int rc = 0;
+    if (hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        launch_voxelize(c->lig, c->d_brot, 1, c->d_bits, c->d_vidx, c->d_lg, c->stream) != hipSuccess)
+        rc = fail("mdk_score_map: voxelization failed");
+    if (!rc) rc = run_fft_pipeline(c, 1, d_map);
+    if (!rc && (hipMemcpyAsync(out, d_map, sizeof(float) * nf3, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail("mdk_score_map: copy back failed");
+    (void)hipFree(d_map);
+    return rc;
+}
+
+int mdk_fft1d(int device, int n, int count, const float *in, float *out, int sign) {
+    if (!in || !out || n < 2 || count < 1) return fail("mdk_fft1d: invalid argument");
+    FftPlan p{};
+    std::vector<float2> tw;
+    if (!make_plan(n, p, tw)) return fail("mdk_fft1d: length has a prime factor above 13");
+    const size_t lds = sizeof(float2) * ((size_t)p.ntw + 2 * (size_t)n * (WZ + 1));
+    if (lds > 160 * 1024) return fail("mdk_fft1d: length too large");
+    HIPCHK(hipSetDevice(device));
+    float2 *d_in = nullptr, *d_out = nullptr, *d_tw = nullptr;
+    const size_t bytes = sizeof(float2) * (size_t)n * count;
+    int rc = 0;
+    if (hipMalloc(&d_in, bytes) != hipSuccess || hipMalloc(&d_out, bytes) != hipSuccess ||
+        hipMalloc(&d_tw, sizeof(float2) * tw.size()) != hipSuccess)
+        rc = fail("mdk_fft1d: allocation failed");
+    if (!rc && (hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail("mdk_fft1d: upload failed");
+    if (!rc && allow_lds(k_lines, lds) != hipSuccess) rc = fail("mdk_fft1d: cannot raise the LDS limit");
+    if (!rc) {
+        k_lines<<<(count + WZ - 1) / WZ, NT, lds>>>(d_in, d_out, p, d_tw, count, sign > 0 ? 1 : 0);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail("mdk_fft1d: kernel failed");
+    }
+    if (!rc && hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("mdk_fft1d: download failed");
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    if (d_tw) (void)hipFree(d_tw);
+    return rc;
+}
+
+}  // extern "C"

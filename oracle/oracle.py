@@ -1,0 +1,359 @@
+"""Python face of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/liboracle.so`` (the C restatement of the reference's CPU path) and,
+when it has been built, ``oracle/_ref/libmdkref.so`` (the reference's own CPU sources
+compiled with a small driver).  Only tests, the smoke check and the benchmark's CPU
+baseline import this module, and only as the checker.
+
+Also holds the oracle's own PDB reader and table lookups (src/pdb_entry.cpp:18-105,
+src/parameter.cpp:425-470) and an FP64 numpy path for the FFT stages.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(os.path.dirname(HERE), "megadock_amd", "data")
+
+
+class SortScore(C.Structure):
+    _fields_ = [("score", C.c_float), ("index", C.c_int * 4)]
+
+
+SORTSCORE_DTYPE = np.dtype([("score", np.float32), ("index", np.int32, (4,))])
+
+_lib = None
+_ref = None
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = C.CDLL(path)
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libmdkref.so"))
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libmdkref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("reference build missing: oracle/_ref/libmdkref.so")
+        _ref = C.CDLL(path)
+    return _ref
+
+
+# ------------------------------------------------------------------ tables / PDB
+def _load_table(name):
+    d = {}
+    with open(os.path.join(DATA, name)) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            k, v = line.rstrip("\n").split("\t")
+            if k not in d:  # std::map::insert keeps the first value
+                d[k] = np.float32(float(v))
+    return d
+
+
+_TABLES = {}
+
+
+def tables():
+    if not _TABLES:
+        _TABLES["radius"] = _load_table("charmm_radius.tsv")
+        _TABLES["charge"] = _load_table("charmm_charge.tsv")
+        _TABLES["ace"] = _load_table("ace.tsv")
+    return _TABLES
+
+
+def load_rotations(n):
+    """float32[n,3] rotation set (src/parameter.cpp:3793-61886)."""
+    rows = []
+    with open(os.path.join(DATA, "rotations_%d.txt" % n)) as f:
+        for line in f:
+            if not line.startswith("#"):
+                rows.append([float(v) for v in line.split()])
+    return np.array(rows, dtype=np.float64).astype(np.float32)
+
+
+def erase_space(s):
+    """src/pdb_entry.cpp:78-105"""
+    while "  " in s:
+        n = s.find("  ")
+        s = s[:n] + s[n + 1:]
+    if s[:1] == " ":
+        s = s[1:]
+    if s[-1:] == " ":
+        s = s[:-1]
+    return s
+
+
+def read_pdb(path):
+    """src/pdb_entry.cpp:18-75 -> (coords float32[n,3], keys list)."""
+    xyz, keys = [], []
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if line[0:6] in ("ATOM  ", "HETATM"):
+                if line[76:78] == " H":
+                    continue
+                xyz.append([float(line[30:38]), float(line[38:46]), float(line[46:54])])
+                keys.append(erase_space(line[11:21]))
+    return np.array(xyz, dtype=np.float64).astype(np.float32).reshape(-1, 3), keys
+
+
+class Protein:
+    """Atoms plus per-atom radius/charge/ACE as src/protein.cpp:19-64 looks them up."""
+
+    def __init__(self, path):
+        self.path = path
+        self.xyz, self.keys = read_pdb(path)
+        t = tables()
+        self.radius = np.array([t["radius"].get(k, np.float32(1.8)) for k in self.keys], np.float32)
+        self.charge = np.array([t["charge"].get(k, np.float32(0.0)) for k in self.keys], np.float32)
+        self.ace = np.array([t["ace"].get(k, np.float32(0.0)) for k in self.keys], np.float32)
+        self.center = center(self.xyz)
+
+    @property
+    def na(self):
+        return len(self.xyz)
+
+
+# ------------------------------------------------------------------ C oracle wrappers
+def rotation_coef(theta):
+    r = np.zeros(9, np.float32)
+    th = np.ascontiguousarray(theta, np.float32)
+    lib().or_rotation_coef(_fp(th), _fp(r))
+    return r
+
+
+def rotate_ligand(xyz, center_, r):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    out = np.zeros_like(xyz)
+    c = np.ascontiguousarray(center_, np.float32)
+    lib().or_rotate_ligand(C.c_int(len(xyz)), _fp(xyz), _fp(c), _fp(np.ascontiguousarray(r, np.float32)), _fp(out))
+    return out
+
+
+def grid_coord(ng, gw):
+    out = np.zeros(ng, np.float32)
+    lib().or_grid_coord(C.c_int(ng), C.c_float(gw), _fp(out))
+    return out
+
+
+def center(xyz):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    c = np.zeros(3, np.float32)
+    lib().or_center(C.c_int(len(xyz)), _fp(xyz), _fp(c))
+    return c
+
+
+def shift_center(xyz):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    out = np.zeros_like(xyz)
+    lib().or_shift_center(C.c_int(len(xyz)), _fp(xyz), _fp(out))
+    return out
+
+
+def ligand_voxel(ng, gw, xyz_rot, radius, charge, delta=1.0):
+    gr = np.zeros(ng ** 3, np.float32)
+    gi = np.zeros(ng ** 3, np.float32)
+    gc = grid_coord(ng, gw)
+    xyz_rot = np.ascontiguousarray(xyz_rot, np.float32)
+    lib().or_ligand_voxel(C.c_int(ng), C.c_float(gw), C.c_int(len(xyz_rot)), _fp(xyz_rot),
+                          _fp(np.ascontiguousarray(radius, np.float32)),
+                          _fp(np.ascontiguousarray(charge, np.float32)), _fp(gc),
+                          C.c_float(delta), _fp(gr), _fp(gi))
+    return gr, gi
+
+
+def receptor_voxel(ng, gw, prot, aceratio=1.0, rec_core=-45.0, eratio=1.0):
+    gr = np.zeros(ng ** 3, np.float32)
+    gi = np.zeros(ng ** 3, np.float32)
+    gc = grid_coord(ng, gw)
+    xyz = shift_center(prot.xyz)
+    lib().or_receptor_voxel(C.c_int(ng), C.c_float(gw), C.c_int(prot.na), _fp(xyz),
+                            _fp(prot.radius), _fp(prot.charge), _fp(prot.ace), _fp(gc),
+                            C.c_float(aceratio), C.c_float(rec_core), C.c_float(eratio),
+                            _fp(gr), _fp(gi))
+    return gr, gi
+
+
+def embed(ng, nf, gr, gi):
+    out = np.zeros(2 * nf ** 3, np.float32)
+    lib().or_embed(C.c_int(ng), C.c_int(nf), _fp(np.ascontiguousarray(gr, np.float32)),
+                   _fp(np.ascontiguousarray(gi, np.float32)), _fp(out))
+    return out.view(np.complex64)
+
+
+def fft3d(nf, data, sign):
+    a = np.ascontiguousarray(data, np.complex64).ravel().copy()
+    lib().or_fft3d(C.c_int(nf), _fp(a.view(np.float32)), _fp(a.view(np.float32)), C.c_int(sign))
+    return a
+
+
+def fft1d(n, lines, sign):
+    a = np.ascontiguousarray(lines, np.complex64).copy()
+    count = a.size // n
+    lib().or_fft1d(C.c_int(n), C.c_int(count), _fp(a.view(np.float32)), _fp(a.view(np.float32)), C.c_int(sign))
+    return a
+
+
+def receptor_fft(ng, nf, gr, gi):
+    nf3 = nf ** 3
+    rr = np.zeros(nf3, np.float32)
+    ri = np.zeros(nf3, np.float32)
+    lib().or_receptor_fft(C.c_int(ng), C.c_int(nf), _fp(gr), _fp(gi), _fp(rr), _fp(ri))
+    return rr, ri
+
+
+def convolution(rr, ri, lig):
+    lig = np.ascontiguousarray(lig, np.complex64)
+    out = np.zeros_like(lig)
+    lib().or_convolution(C.c_size_t(len(rr)), _fp(rr), _fp(ri), _fp(lig.view(np.float32)),
+                         _fp(out.view(np.float32)))
+    return out
+
+
+def score_sort(nf, t, fftout, rot):
+    out = np.zeros(t, SORTSCORE_DTYPE)
+    fo = np.ascontiguousarray(fftout, np.complex64)
+    lib().or_score_sort(C.c_int(nf), C.c_int(t), _fp(fo.view(np.float32)), C.c_int(rot),
+                        C.c_void_p(out.ctypes.data))
+    return out
+
+
+def sort_topn(top, n):
+    top = np.ascontiguousarray(top, SORTSCORE_DTYPE).copy()
+    lib().or_sort_topn(C.c_void_p(top.ctypes.data), C.c_int(len(top)), C.c_int(n))
+    return top[:n]
+
+
+def dock(ng, nf, gw, lig, rr, ri, angles, rot_ids, t, threads=1, delta=1.0):
+    rot_ids = np.ascontiguousarray(rot_ids, np.int32)
+    out = np.zeros(len(rot_ids) * t, SORTSCORE_DTYPE)
+    ang = np.ascontiguousarray(angles, np.float32).ravel()
+    rc = lib().or_dock(C.c_int(ng), C.c_int(nf), C.c_float(gw), C.c_int(lig.na), _fp(lig.xyz),
+                       _fp(lig.center), _fp(lig.radius), _fp(lig.charge), C.c_float(delta),
+                       _fp(rr), _fp(ri), _fp(ang), _ip(rot_ids), C.c_int(len(rot_ids)), C.c_int(t),
+                       C.c_int(threads), C.c_void_p(out.ctypes.data))
+    assert rc == 0
+    return out
+
+
+# ------------------------------------------------------------------ FP64 path (numpy)
+def correlation_fp64(nf, rec_grid_c, lig_grid_c):
+    """Re(IFFT(conj(FFT r) * FFT l)) in float64 for complex nf^3 grids (unnormalised)."""
+    R = np.fft.fftn(np.asarray(rec_grid_c, np.complex128).reshape(nf, nf, nf))
+    L = np.fft.fftn(np.asarray(lig_grid_c, np.complex128).reshape(nf, nf, nf))
+    return np.real(np.fft.ifftn(np.conj(R) * L)) * nf ** 3
+
+
+# ------------------------------------------------------------------ reference build
+def ref_load(rec_pdb, lig_pdb, extra=()):
+    r = ref()
+    arr = (C.c_char_p * max(1, len(extra)))(*[e.encode() for e in extra])
+    return r.ref_load(rec_pdb.encode(), lig_pdb.encode(), C.c_int(len(extra)), arr)
+
+
+def ref_protein(which):
+    r = ref()
+    na = r.ref_num_atoms(C.c_int(which))
+    xyz = np.zeros((na, 3), np.float32)
+    r.ref_coords(C.c_int(which), _fp(xyz))
+    rad = np.zeros(na, np.float32)
+    chg = np.zeros(na, np.float32)
+    ace = np.zeros(na, np.float32)
+    r.ref_atom_params(C.c_int(which), _fp(rad), _fp(chg), _fp(ace))
+    c = np.zeros(3, np.float32)
+    r.ref_center(C.c_int(which), _fp(c))
+    keys = []
+    buf = C.create_string_buffer(64)
+    for i in range(na):
+        r.ref_atom_type(C.c_int(which), C.c_int(i), buf, C.c_int(64))
+        keys.append(buf.value.decode())
+    return dict(xyz=xyz, radius=rad, charge=chg, ace=ace, center=c, keys=keys)
+
+
+def ref_receptor_voxel(ng, gw, aceratio=1.0, rec_core=-45.0, lig_core=1.0, eratio=1.0):
+    gr = np.zeros(ng ** 3, np.float32)
+    gi = np.zeros(ng ** 3, np.float32)
+    ref().ref_receptor_voxel(C.c_int(ng), C.c_float(gw), C.c_float(aceratio), C.c_float(rec_core),
+                             C.c_float(lig_core), C.c_float(eratio), _fp(gr), _fp(gi))
+    return gr, gi
+
+
+def ref_ligand_voxel(ng, gw, xyz_rot, delta=1.0, eratio=1.0):
+    gr = np.zeros(ng ** 3, np.float32)
+    gi = np.zeros(ng ** 3, np.float32)
+    xyz_rot = np.ascontiguousarray(xyz_rot, np.float32)
+    ref().ref_ligand_voxel(C.c_int(ng), C.c_float(gw), _fp(xyz_rot), C.c_float(1.0), C.c_float(-45.0),
+                           C.c_float(delta), C.c_float(eratio), _fp(gr), _fp(gi))
+    return gr, gi
+
+
+def ref_rotate(xyz, theta):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    out = np.zeros_like(xyz)
+    th = np.ascontiguousarray(theta, np.float32)
+    ref().ref_rotate(_fp(xyz), C.c_int(len(xyz)), _fp(th), _fp(out))
+    return out
+
+
+# ------------------------------------------------------------------ A16 grid sizing
+GRID_TABLE_11 = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 18, 20, 21, 22, 24, 25, 27,
+                 28, 30, 32, 33, 35, 36, 40, 42, 44, 45, 48, 49, 50, 54, 55, 56, 60, 63,
+                 64, 66, 70, 72, 75, 77, 80, 81, 84, 88, 90, 96, 98, 99, 100, 105, 108,
+                 110, 112, 120, 121, 125, 126, 128, 132, 135, 140, 144, 147, 150,
+                 154, 160, 162, 165, 168, 175, 176, 180, 189, 192, 196, 198, 200,
+                 210, 216, 220, 224, 225, 231, 240, 242, 243, 245, 250, 252, 256]
+
+
+def autogrid(rec, lig, gw=1.2, table=GRID_TABLE_11):
+    """src/control.cpp:287-379 (11-base table, src/control.cpp:91-125, truncated at 256):
+    returns ng = max(receptor ng, ligand ng)."""
+    gw = np.float32(gw)
+    edge_lo = rec.xyz.min(axis=0)
+    edge_hi = rec.xyz.max(axis=0)
+    size_rec = np.float32(0.0)
+    for i in range(3):
+        s = np.float32(edge_hi[i] - edge_lo[i])
+        if s > size_rec:
+            size_rec = s
+    size_rec = np.float32(np.float64(size_rec) + 2.0 * 8.0)
+    ngr = 1 + int(np.float32(size_rec / gw))
+    x = lig.xyz
+    d2max = np.float32(0.0)
+    for i in range(len(x) - 1):
+        d = x[i + 1:] - x[i]
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        m = d2.max()
+        if m > d2max:
+            d2max = m
+    size_lig = np.float32(np.sqrt(d2max))
+    size_lig = np.float32(np.float64(size_lig) + 2.0 * 3.0)
+    ngl = 1 + int(np.float32(size_lig / gw))
+
+    def pick(n):
+        for v in table:
+            if v >= n:
+                return v
+        return n
+    return max(pick(ngr), pick(ngl))

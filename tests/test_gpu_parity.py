@@ -1,0 +1,191 @@
+"""GPU parity: the HIP path (through the libmdk C ABI) against the CPU oracle.
+
+Bit-exact for voxel grids; FP32 transforms against float64 numpy within the stated
+tolerances; per-rotation best scores within 1e-4 relative of the oracle, best
+positions identical except at near-ties judged on the FP64 score map.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import megadock_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FFT_SIZES = [8, 12, 16, 18, 20, 22, 26, 28, 36, 40, 42, 44, 48, 64, 66, 70, 72, 80, 88, 90,
+             96, 98, 100, 110, 128, 144, 154, 160, 192, 256]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    n = M.device_count()
+    assert n >= 1, "no GPU visible"
+    return 0
+
+
+@pytest.fixture(scope="module")
+def gcq(golden_dir):
+    rec = O.Protein(os.path.join(golden_dir, "1gcq_r.pdb"))
+    lig = O.Protein(os.path.join(golden_dir, "1gcq_l.pdb"))
+    ng = O.autogrid(rec, lig)
+    return rec, lig, ng
+
+
+@pytest.mark.parametrize("n", FFT_SIZES)
+def test_fft1d_matches_numpy(gpu, n):
+    _say("fft1d n=%d" % n)
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((37, n)) + 1j * rng.standard_normal((37, n))).astype(np.complex64)
+    for sign in (-1, 1):
+        y = M.fft1d(x, sign)
+        ref = np.fft.fft(x.astype(np.complex128), axis=-1) if sign < 0 else np.fft.ifft(x.astype(np.complex128), axis=-1) * n
+        err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        assert err < 5e-7 * max(1.0, np.log2(n)), (n, sign, err)
+
+
+def _lig_ctx(lig, ng, t=1, max_batch=16):
+    nf = 2 * ng
+    ctx = M.Context(nf, ng, lig.na, t=t, max_batch=max_batch)
+    ctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, O.grid_coord(ng, 1.2), 1.2)
+    return ctx
+
+
+def test_voxelize_bit_exact_1gcq(gpu, gcq):
+    rec, lig, ng = gcq
+    _say("voxelize 1gcq ng=%d" % ng)
+    ctx = _lig_ctx(lig, ng)
+    ang = O.load_rotations(3600)
+    rots = list(range(0, 3600, 50))
+    for b0 in range(0, len(rots), 16):
+        ids = rots[b0:b0 + 16]
+        rc = M.rotation_coefs(ang[ids])
+        gr, gi = ctx.voxelize(rc)
+        for q, a in enumerate(ids):
+            xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+            er, ei = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+            assert np.array_equal(gr[q], er), ("grid_r", a)
+            assert np.array_equal(gi[q], ei), ("grid_i", a)
+    ctx.close()
+
+
+@pytest.mark.parametrize("delta,ng", [(2.0, None), (1.0, 24), (-3.5, 30)])
+def test_voxelize_bit_exact_edge(gpu, gcq, delta, ng):
+    """core value != 1 (the extra surface sweep) and grids too small for the ligand
+    (atoms outside the box: clamped charges, truncated search boxes)."""
+    rec, lig, ng0 = gcq
+    ng = ng or ng0
+    _say("voxelize edge delta=%g ng=%d" % (delta, ng))
+    nf = 2 * ng
+    ctx = M.Context(nf, ng, lig.na, t=1, max_batch=8)
+    ctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, O.grid_coord(ng, 1.2), 1.2, delta)
+    ang = O.load_rotations(360)
+    ids = [0, 7, 99, 200, 311, 359]
+    gr, gi = ctx.voxelize(M.rotation_coefs(ang[ids]))
+    for q, a in enumerate(ids):
+        xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+        er, ei = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge, delta)
+        assert np.array_equal(gr[q], er), ("grid_r", a)
+        assert np.array_equal(gi[q], ei), ("grid_i", a)
+    ctx.close()
+
+
+def test_receptor_spectrum_vs_fp64(gpu, gcq):
+    rec, lig, ng = gcq
+    nf = 2 * ng
+    _say("receptor spectrum nf=%d" % nf)
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ctx = _lig_ctx(lig, ng)
+    ctx.set_receptor(gr, gi)
+    re, im = ctx.receptor_spectrum()
+    emb = O.embed(ng, nf, gr, gi).astype(np.complex128).reshape(nf, nf, nf)
+    ref = np.fft.fftn(emb).ravel()
+    got = re.astype(np.float64) + 1j * im.astype(np.float64)
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    _say("  rel L2 err %.3e" % rel)
+    assert rel < 2e-6
+    ctx.close()
+
+
+def test_score_map_vs_fp64(gpu, gcq):
+    rec, lig, ng = gcq
+    nf = 2 * ng
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ctx = _lig_ctx(lig, ng)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(3600)
+    remb = O.embed(ng, nf, gr, gi)
+    for a in (0, 1234, 3599):
+        _say("score map rotation %d" % a)
+        smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
+        xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+        lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+        ref = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+        err = np.abs(smap - ref).max() / np.abs(ref).max()
+        _say("  max err / max|score| = %.3e" % err)
+        assert err < 2e-6
+    ctx.close()
+
+
+def _near_tie_ok(smap64, i_gpu, i_ref, tol):
+    return abs(smap64[i_gpu] - smap64[i_ref]) <= tol * abs(smap64).max()
+
+
+@pytest.mark.parametrize("t", [1, 3])
+def test_dock_vs_oracle(gpu, gcq, t):
+    rec, lig, ng = gcq
+    nf = 2 * ng
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rr, ri = O.receptor_fft(ng, nf, gr, gi)
+    ctx = _lig_ctx(lig, ng, t=t)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(3600)
+    ids = np.arange(0, 3600, 150, dtype=np.int32)
+    _say("dock %d rotations t=%d nf=%d" % (len(ids), t, nf))
+    got = np.concatenate([ctx.dock(M.rotation_coefs(ang[ids[q:q + 16]]), ids[q:q + 16])
+                          for q in range(0, len(ids), 16)])
+    ref = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, t, threads=8)
+    remb = O.embed(ng, nf, gr, gi)
+    nf3 = nf ** 3
+    for q in range(len(ids) * t):
+        g, r = got[q], ref[q]
+        assert g["index"][0] == r["index"][0]
+        assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
+        if not np.array_equal(g["index"], r["index"]):
+            a = int(ids[q // t])
+            xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+            lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+            s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+            ig = (g["index"][1] * nf + g["index"][2]) * nf + g["index"][3]
+            ir = (r["index"][1] * nf + r["index"][2]) * nf + r["index"][3]
+            assert _near_tie_ok(s64, ig, ir, 2e-6), (q, g, r, s64[ig] / nf3, s64[ir] / nf3)
+    ctx.close()
+
+
+def test_dock_range_matches_batches_and_topn(gpu, gcq):
+    rec, lig, ng = gcq
+    nf = 2 * ng
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ctx = _lig_ctx(lig, ng, t=2, max_batch=32)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(360)
+    rc = M.rotation_coefs(ang)
+    ctx.set_rotations(rc)
+    ctx.dock_range(0, 360)
+    ctx.sync()
+    res = ctx.results(0, 360)
+    _say("dock_range 360 rotations")
+    # same kernels, different batch composition: bitwise identical per rotation
+    ids = np.arange(100, 117, dtype=np.int32)
+    b = ctx.dock(rc[ids], ids)
+    assert np.array_equal(b.view(np.uint8), res[100 * 2:117 * 2].view(np.uint8))
+    top = ctx.topn(360, 50)
+    order = sorted(range(len(res)), key=lambda i: (-res[i]["score"], i))[:50]
+    assert np.array_equal(top.view(np.uint8), res[order].view(np.uint8))
+    ctx.close()
