@@ -459,7 +459,7 @@ static void fft_line(const fft_plan *p, float *x, float *y, int sign)
             for (int r = 1; r < R; r++) {
                 const float xr = x[2 * (j + r * nr)], xi = x[2 * (j + r * nr) + 1];
                 const float wr = tw[2 * ((r - 1) * Ns + k)];
-                const float wi = sign * tw[2 * ((r - 1) * Ns + k) + 1];
+                const float wi = -sign * tw[2 * ((r - 1) * Ns + k) + 1]; /* stored twiddles are the forward ones */
                 v[2 * r] = xr * wr - xi * wi;
                 v[2 * r + 1] = xr * wi + xi * wr;
             }
