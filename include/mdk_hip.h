@@ -99,6 +99,12 @@ int mdk_topn(mdk_ctx *ctx, int nrot, int n, mdk_sortscore *out);
  * stream; *ms = mean milliseconds per repetition. */
 int mdk_time_range(mdk_ctx *ctx, int first, int count, int reps, float *ms);
 
+/* Measurement hook: dock rotations [first, first+count) (count <= max_batch) as one
+ * batch with HIP events between the stages (voxelize, forward z, forward y,
+ * x + correlation, inverse y, inverse z + top-t, per-rotation reduce); ms[q] gets each
+ * stage's milliseconds and *nstages their number (7). No reference counterpart. */
+int mdk_profile_stages(mdk_ctx *ctx, int first, int count, float *ms, int *nstages);
+
 /* Test hook: ligand voxel grids for B rotations (B*ng^3 floats each). */
 int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float *grid_i);
 

@@ -20,9 +20,15 @@ ABI_SYMBOLS = [
     "mdk_version", "mdk_last_error", "mdk_rotation_coef", "mdk_device_count", "mdk_create", "mdk_destroy",
     "mdk_set_receptor", "mdk_set_receptor_spectrum", "mdk_get_receptor_spectrum",
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
-    "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_voxelize", "mdk_score_map",
-    "mdk_fft1d",
+    "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
+    "mdk_score_map", "mdk_fft1d",
 ]
+# every symbol declared in include/mdk_host.h
+HOST_SYMBOLS = [
+    "mdkh_protein_load", "mdkh_protein_free", "mdkh_protein_size", "mdkh_protein_atoms",
+    "mdkh_autogrid", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_last_error",
+]
+STAGES = ["voxelize", "fwd_z", "fwd_y", "x_corr", "inv_y", "inv_z_top", "reduce"]
 
 _lib = None
 
@@ -39,8 +45,9 @@ def lib():
             raise MdkError("libmdk.so not built (run __graft_entry__.build() or make -C megadock_amd/csrc)")
         L = C.CDLL(LIB_PATH)
         L.mdk_last_error.restype = C.c_char_p
+        L.mdkh_last_error.restype = C.c_char_p
         L.mdk_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
-        for name in ABI_SYMBOLS:
+        for name in ABI_SYMBOLS + HOST_SYMBOLS:
             getattr(L, name)  # raises AttributeError if an export is missing
         _lib = L
     return _lib
@@ -156,6 +163,13 @@ class Context:
                "mdk_time_range")
         return ms.value
 
+    def profile_stages(self, first, count):
+        ms = np.zeros(16, np.float32)
+        n = C.c_int(0)
+        _check(lib().mdk_profile_stages(self.h, C.c_int(first), C.c_int(count), _fp(ms), C.byref(n)),
+               "mdk_profile_stages")
+        return dict(zip(STAGES, [float(v) for v in ms[:n.value]]))
+
     def voxelize(self, rotcoef):
         rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
         B = len(rc)
@@ -179,3 +193,67 @@ def fft1d(lines, sign=-1, device=0):
     _check(lib().mdk_fft1d(C.c_int(device), C.c_int(n), C.c_int(count), _fp(a.view(np.float32)),
                            _fp(out.view(np.float32)), C.c_int(sign)), "mdk_fft1d")
     return out
+
+
+# ---------------------------------------------------------------- host layer
+def _hcheck(rc, what):
+    if rc != 0:
+        raise MdkError("%s failed: %s" % (what, lib().mdkh_last_error().decode()))
+
+
+class Protein:
+    """PDB atoms with CHARMM radius/charge and ACE per atom (mdkh_protein_*)."""
+
+    def __init__(self, path):
+        self.path = path
+        h = C.c_void_p()
+        _hcheck(lib().mdkh_protein_load(path.encode(), C.byref(h)), "mdkh_protein_load")
+        self.h = h
+        n = C.c_int(0)
+        _hcheck(lib().mdkh_protein_size(h, C.byref(n)), "mdkh_protein_size")
+        na = n.value
+        self.xyz = np.zeros((na, 3), np.float32)
+        self.radius = np.zeros(na, np.float32)
+        self.charge = np.zeros(na, np.float32)
+        self.ace = np.zeros(na, np.float32)
+        self.center = np.zeros(3, np.float32)
+        _hcheck(lib().mdkh_protein_atoms(h, _fp(self.xyz), _fp(self.radius), _fp(self.charge), _fp(self.ace),
+                                         _fp(self.center)), "mdkh_protein_atoms")
+
+    @property
+    def na(self):
+        return len(self.xyz)
+
+    def __del__(self):
+        try:
+            lib().mdkh_protein_free(self.h)
+        except Exception:
+            pass
+
+
+def autogrid(rec, lig, gw=1.2, table=11):
+    ng = C.c_int(0)
+    _hcheck(lib().mdkh_autogrid(rec.h, lig.h, C.c_float(gw), C.c_int(table), C.byref(ng)), "mdkh_autogrid")
+    return ng.value
+
+
+def grid_coord(ng, gw=1.2):
+    out = np.zeros(ng, np.float32)
+    _hcheck(lib().mdkh_grid_coord(C.c_int(ng), C.c_float(gw), _fp(out)), "mdkh_grid_coord")
+    return out
+
+
+def receptor_grids(rec, ng, gw=1.2, aceratio=1.0, rec_core=-45.0, eratio=1.0):
+    gr = np.zeros(ng ** 3, np.float32)
+    gi = np.zeros(ng ** 3, np.float32)
+    _hcheck(lib().mdkh_receptor_voxel(rec.h, C.c_int(ng), C.c_float(gw), C.c_float(aceratio),
+                                      C.c_float(rec_core), C.c_float(eratio), _fp(gr), _fp(gi)),
+            "mdkh_receptor_voxel")
+    return gr, gi
+
+
+def run(args):
+    """The megadock command line (e.g. ["-R", r, "-L", l, "-o", out])."""
+    argv = ["megadock-hip"] + list(args)
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    _hcheck(lib().mdkh_run(C.c_int(len(argv)), arr), "mdkh_run")

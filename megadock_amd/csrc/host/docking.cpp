@@ -1,0 +1,218 @@
+// Rotation-loop driver and decoy output of the drop-in (src/docking.cpp,
+// src/fft_process.cpp, src/control.cpp:18-88, :468-529).  The loop itself runs on
+// the GPU through the libmdk C ABI; this file prepares the receptor, hands the
+// rotation table and ligand to the device, and writes the .out file.
+#pragma clang fp contract(off)
+
+#include <sys/time.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "mdk_host.h"
+
+namespace mdk {
+
+namespace {
+
+double now() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+std::string abi_error(const char *what) { return std::string(what) + ": " + mdk_last_error(); }
+
+// rotations per device batch: about 2 GiB of per-batch work buffers, at most 128
+int auto_batch(int nf, int ng, int t) {
+    const double H = nf / 2 + 1;
+    const double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
+                                  (double)nf * nf * t);
+    int b = (int)(2.0 * 1024 * 1024 * 1024 / per_rot);
+    return std::max(1, std::min(128, b));
+}
+
+const float PI_F = (float)(4.0 * std::atan(1.0));  // src/constant.h:24
+
+}  // namespace
+
+// ------------------------------------------------------------------ FFTProcess
+FFTProcess::~FFTProcess() {
+    if (ctx_) mdk_destroy(ctx_);
+}
+
+int FFTProcess::alloc_array(int num_fft, int device, std::string &err) {
+    num_fft_ = num_fft;
+    const int ng = num_fft / 2;
+    int batch = prm_->batch > 0 ? prm_->batch : auto_batch(num_fft, ng, prm_->num_sort);
+    batch = std::min(batch, std::max(1, prm_->num_rot_angles));
+    if (mdk_create(device, num_fft, ng, lig_->num_atoms(), prm_->num_sort, batch, &ctx_)) {
+        err = abi_error("mdk_create");
+        return 1;
+    }
+    const int toprank = std::max(prm_->num_rot_angles * prm_->num_sort, prm_->num_output);
+    top_.assign(toprank, SortScore{0.0f, {0, 0, 0, 0}});
+    return 0;
+}
+
+int FFTProcess::receptor_fft(const float *grid_r, const float *grid_i, std::string &err) {
+    if (mdk_set_receptor(ctx_, grid_r, grid_i)) {
+        err = abi_error("mdk_set_receptor");
+        return 1;
+    }
+    return 0;
+}
+
+int FFTProcess::ligand_data_transfer_gpu(const std::vector<float> &gc, std::string &err) {
+    std::vector<float> rot(9 * (size_t)prm_->num_rot_angles);
+    if (mdk_rotation_coef(prm_->zangle.data(), prm_->num_rot_angles, rot.data()) ||
+        mdk_set_rotations(ctx_, rot.data(), prm_->num_rot_angles)) {
+        err = abi_error("rotation table");
+        return 1;
+    }
+    if (mdk_set_ligand(ctx_, lig_->coords().data(), lig_->center(), lig_->radius.data(), lig_->charge.data(),
+                       gc.data(), prm_->grid_width, prm_->lig_core)) {
+        err = abi_error("mdk_set_ligand");
+        return 1;
+    }
+    return 0;
+}
+
+int FFTProcess::dock_rotations(int first, int count, std::string &err) {
+    if (mdk_dock_range(ctx_, first, count)) {
+        err = abi_error("mdk_dock_range");
+        return 1;
+    }
+    return 0;
+}
+
+void FFTProcess::top_score_clean() {
+    for (auto &s : top_) s.score = 0.0f;
+}
+
+int FFTProcess::sort_index(std::string &err) {
+    const int no = std::min((int)top_.size(), prm_->num_output);
+    std::vector<mdk_sortscore> best(no);
+    if (mdk_topn(ctx_, prm_->num_rot_angles, no, best.data())) {
+        err = abi_error("mdk_topn");
+        return 1;
+    }
+    std::memcpy(top_.data(), best.data(), sizeof(SortScore) * no);
+    return 0;
+}
+
+// ------------------------------------------------------------------ Docking
+int Docking::initialize(int device, std::string &err) {
+    num_grid_ = std::max(rec_->num_grid(), lig_->num_grid());
+    prm_->num_grid = num_grid_;
+    prm_->num_fft = 2 * num_grid_;
+    fft_.reset(new FFTProcess(prm_, rec_, lig_));
+    return fft_->alloc_array(2 * num_grid_, device, err);
+}
+
+int Docking::rec_init(std::string &err) {
+    const double t0 = now();
+    const int ng = num_grid_;
+    grid_coord_ = grid_coord(ng, prm_->grid_width);
+    std::vector<float> xyz(3 * (size_t)rec_->num_atoms());
+    rec_->shift_center(xyz.data());
+    const size_t ng3 = (size_t)ng * ng * ng;
+    std::vector<float> gr(ng3), gi(ng3);
+    rec_->voxelize(ng, prm_->grid_width, grid_coord_, xyz.data(), prm_->ace_ratio, prm_->rec_core,
+                   prm_->elec_ratio, gr.data(), gi.data());
+    if (fft_->receptor_fft(gr.data(), gi.data(), err)) return 1;
+    t_receptor = now() - t0;
+    return 0;
+}
+
+int Docking::dockz(std::string &err) {
+    const double t0 = now();
+    fft_->top_score_clean();
+    std::printf("\nLigand = %s\nTarget receptors:\n %s\n\n", prm_->lig_file.c_str(), prm_->rec_file.c_str());
+    if (fft_->ligand_data_transfer_gpu(grid_coord_, err)) return 1;
+    const int R = prm_->num_rot_angles;
+    const int chunk = prm_->progress ? std::max(R / 10, 1) : R;
+    for (int a = 0; a < R; a += chunk) {
+        const int n = std::min(chunk, R - a);
+        if (fft_->dock_rotations(a, n, err)) return 1;
+        if (prm_->progress) {
+            if (mdk_sync(fft_->ctx())) {
+                err = abi_error("mdk_sync");
+                return 1;
+            }
+            std::printf("   >Ligand rotation = %5d / %5d\n", a + n, R);
+        }
+    }
+    if (fft_->sort_index(err)) return 1;
+    t_docking = now() - t0;
+    return 0;
+}
+
+int Docking::output(std::string &err) {
+    FILE *f = std::fopen(prm_->out_file.c_str(), "w");
+    if (!f) {
+        err = "[ERROR] Output file [" + prm_->out_file + "] not open!!";
+        return 1;
+    }
+    std::fprintf(f, "%d\t%.2f\n", 2 * num_grid_, prm_->grid_width);  // src/docking.cpp:413-414
+    std::fprintf(f, "\t%f\t%f\t%f\n", 0.0, 0.0, 0.0);                 // receptor angle
+    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm_->rec_file.c_str(), rec_->center(0), rec_->center(1), rec_->center(2));
+    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm_->lig_file.c_str(), lig_->center(0), lig_->center(1), lig_->center(2));
+    for (int rank = 0; rank < prm_->num_output; rank++) {  // src/docking.cpp:434-448
+        const int n = fft_->top_index(rank, 0);
+        float th[3];
+        for (int k = 0; k < 3; k++) {
+            th[k] = prm_->zangle[3 * n + k];
+            if (th[k] >= PI_F) th[k] = (float)(th[k] - PI_F * 2.0);
+        }
+        std::fprintf(f, "%f\t%f\t%f\t%d\t%d\t%d\t%.2f\n", th[0], th[1], th[2], fft_->top_index(rank, 1),
+                     fft_->top_index(rank, 2), fft_->top_index(rank, 3), fft_->top_score(rank));
+    }
+    std::fclose(f);
+    return 0;
+}
+
+// ------------------------------------------------------------------ run
+int run(int argc, char **argv, std::string &err) {
+    const double t0 = now();
+    Parameter prm;
+    const int rc = prm.initialize(argc, argv, err);
+    if (rc) return rc;
+    int ndev = 0;
+    if (mdk_device_count(&ndev) || ndev < 1) {
+        err = "no GPU visible: this build runs the rotation loop on the GPU";
+        return 1;
+    }
+    if (prm.gpu_limit == 0) {
+        err = "-G 0 asks for the CPU loop, which this build does not provide";
+        return 1;
+    }
+    std::printf("# Using %3d GPU\n", 1);
+    Receptor rec(prm.rec_file);
+    Ligand lig(prm.lig_file);
+    if (rec.initialize(prm, err) || lig.initialize(prm, err)) return 1;
+    if (!prm.num_fft_flag) {
+        const int ngr = autogrid_receptor(rec, prm), ngl = autogrid_ligand(lig, prm);
+        rec.num_grid(ngr);
+        lig.num_grid(ngl);
+        std::printf("\nReceptor = %s\nNumber of grid = %d\nFFT N = %d\n", prm.rec_file.c_str(), ngr, 2 * ngr);
+        std::printf("\nLigand = %s\nNumber of grid = %d\nFFT N = %d\n", prm.lig_file.c_str(), ngl, 2 * ngl);
+    } else {
+        if (check_grid(rec, lig, prm, err)) return 1;
+        rec.num_grid(prm.num_grid);
+        lig.num_grid(prm.num_grid);
+    }
+    Docking dock(&prm, &rec, &lig);
+    if (dock.initialize(0, err)) return 1;
+    const double t_init = now() - t0;
+    std::printf("\n---------- Start docking calculations\n");
+    if (dock.rec_init(err) || dock.dockz(err) || dock.output(err)) return 1;
+    std::printf("\nElapsed time  = %8.2f sec\n", now() - t0);
+    std::printf(" Initialization = %8.2f sec\n Receptor process = %8.2f sec\n Docking = %8.2f sec (%d rotations, %.1f rotations/s)\n",
+                t_init, dock.t_receptor, dock.t_docking, prm.num_rot_angles, prm.num_rot_angles / dock.t_docking);
+    return 0;
+}
+
+}  // namespace mdk

@@ -28,6 +28,7 @@
 #include "../../include/mdk_hip.h"
 
 #define MDK_MAXT 16
+#define MDK_NSTAGE 7  // voxelize, fwd_z, fwd_y, x_corr, inv_y, inv_z_top, reduce
 
 namespace {
 
@@ -461,27 +462,38 @@ size_t lds_bytes(const mdk_ctx *c, int lines) {
 
 // Run the forward ligand passes + correlation + inverse passes for B rotations whose
 // voxel grids are already in d_lg; output top-t candidates (or the score map).
-int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev) {
+// ev (optional): events recorded after each of the five FFT-side stages
+int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     hipStream_t s = c->stream;
     const int lines_z = B * ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
     k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, B), NT, lds_bytes(c, WY), s>>>(c->d_a1, c->d_a2, c->plan, c->d_tw, ng, nd);
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
     k_x_corr<<<dim3(c->Hp / WX, nf, B), NT, lds_bytes(c, 2 * WX), s>>>(c->d_a2, c->d_rec, c->d_cc, c->plan, c->d_tw, ng, nd, c->H, c->Hp);
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
     k_inv_y<<<dim3(c->Hp / WY, nf, B), NT, lds_bytes(c, WY), s>>>(c->d_cc, c->plan, c->d_tw, c->Hp);
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
     const int rows = B * nf * nf;
     const size_t lds_z = lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * c->t;
     k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(c->d_cc, c->d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, c->t,
                                                        (float)((size_t)nf * nf * nf));
+    if (ev) HIPCHK(hipEventRecord(ev[4], s));
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev) {
+// ev (optional): MDK_NSTAGE + 1 events bracketing the stages of one batch
+int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev,
+               hipEvent_t *ev = nullptr) {
+    if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
     HIPCHK(launch_voxelize(c->lig, rot_dev, B, c->d_bits, c->d_vidx, c->d_lg, c->stream));
-    if (run_fft_pipeline(c, B, nullptr)) return 1;
+    if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+    if (run_fft_pipeline(c, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
     k_reduce_top<<<B, NT, 0, c->stream>>>(c->d_cand, out_dev, id_dev, nf * nf, c->t, nf, (float)((size_t)nf * nf * nf));
+    if (ev) HIPCHK(hipEventRecord(ev[7], c->stream));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -763,6 +775,25 @@ int mdk_time_range(mdk_ctx *c, int first, int count, int reps, float *ms) {
     HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = t / reps;
     return 0;
+}
+
+int mdk_profile_stages(mdk_ctx *c, int first, int count, float *ms, int *nstages) {
+    if (!c || !ms || !nstages) return fail("mdk_profile_stages: null argument");
+    if (!c->have_rec || !c->have_lig || !c->d_rot) return fail("mdk_profile_stages: receptor, ligand and rotations must be set first");
+    if (first < 0 || count < 1 || count > c->max_batch || first + count > c->nrot)
+        return fail("mdk_profile_stages: count must be 1..max_batch inside the rotation table");
+    HIPCHK(hipSetDevice(c->device));
+    hipEvent_t ev[MDK_NSTAGE + 1];
+    int made = 0, rc = 0;
+    for (; made < MDK_NSTAGE + 1; made++)
+        if (hipEventCreate(&ev[made]) != hipSuccess) { rc = fail("mdk_profile_stages: event creation failed"); break; }
+    if (!rc) rc = dock_batch(c, c->d_rot + 9 * (size_t)first, c->d_rotid + first, count, c->d_results + (size_t)first * c->t, ev);
+    if (!rc && hipEventSynchronize(ev[MDK_NSTAGE]) != hipSuccess) rc = fail("mdk_profile_stages: sync failed");
+    for (int q = 0; !rc && q < MDK_NSTAGE; q++)
+        if (hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]) != hipSuccess) rc = fail("mdk_profile_stages: timing failed");
+    for (int q = 0; q < made; q++) (void)hipEventDestroy(ev[q]);
+    *nstages = MDK_NSTAGE;
+    return rc;
 }
 
 int mdk_voxelize(mdk_ctx *c, const float *rotcoef, int B, float *grid_r, float *grid_i) {

@@ -358,6 +358,8 @@ void or_embed(int ng, int nf, const float *grid_r, const float *grid_i, float *c
  * Stage with radix R and span Ns: for j < n/R, k = j mod Ns,
  *   v_r = x[j + r n/R] * w^(r k), w = exp(sign 2 pi i / (Ns R)),
  *   y[(j/Ns) Ns R + k + m Ns] = sum_r v_r exp(sign 2 pi i r m / R). */
+static void dft_table(int R);
+
 typedef struct {
     int n, nfac, fac[32];
     float *tw; /* per stage: (R-1)*Ns complex twiddles, concatenated */
@@ -384,6 +386,7 @@ static fft_plan *plan_create(int n)
     int Ns = 1;
     for (int s = 0; s < p->nfac; s++) { ntw += (size_t)(p->fac[s] - 1) * Ns; Ns *= p->fac[s]; }
     p->tw = (float *)malloc(sizeof(float) * 2 * (ntw + 1));
+    for (int s = 0; s < p->nfac; s++) dft_table(p->fac[s]);
     Ns = 1;
     size_t o = 0;
     for (int s = 0; s < p->nfac; s++) {
@@ -406,6 +409,23 @@ static void plan_destroy(fft_plan *p)
     free(p);
 }
 
+/* cos/sin(2 pi e / R) for e < R, per radix (filled once; radices up to 64) */
+static float g_dft_c[65][64], g_dft_s[65][64];
+static int g_dft_ready[65];
+
+static void dft_table(int R)
+{
+    if (R > 64 || g_dft_ready[R]) return;
+#pragma omp critical(mdk_oracle_dft_table)
+    if (!g_dft_ready[R]) {
+        for (int e = 0; e < R; e++) {
+            g_dft_c[R][e] = (float)cos(2.0 * M_PI * e / R);
+            g_dft_s[R][e] = (float)sin(2.0 * M_PI * e / R);
+        }
+        g_dft_ready[R] = 1;
+    }
+}
+
 /* small R-point DFT in place on v (interleaved), sign -1/+1 */
 static void small_dft(int R, float *v, int sign)
 {
@@ -426,17 +446,14 @@ static void small_dft(int R, float *v, int sign)
         v[6] = a1r - tr; v[7] = a1i - ti;
         return;
     }
+    const float *ct = g_dft_c[R], *st = g_dft_s[R];
     for (int m = 0; m < R; m++) {
-        double sr = 0, si = 0;
         float accr = v[0], acci = v[1];
-        for (int r = 1; r < R; r++) {
-            const int e = (r * m) % R;
-            const double a = sign * 2.0 * M_PI * e / R;
-            const float c = (float)cos(a), s = (float)sin(a);
+        for (int r = 1, e = m; r < R; r++, e = (e + m) % R) {
+            const float c = ct[e], s = sign * st[e];
             accr += v[2 * r] * c - v[2 * r + 1] * s;
             acci += v[2 * r] * s + v[2 * r + 1] * c;
         }
-        (void)sr; (void)si;
         out[2 * m] = accr;
         out[2 * m + 1] = acci;
     }
@@ -491,30 +508,43 @@ int or_fft1d(int n, int count, const float *in, float *out, int sign)
     return 0;
 }
 
-/* 3D transform on a caller-owned plan; along z (contiguous), y, then x. */
+/* 3D transform on a caller-owned plan; along z (contiguous), y, then x.  The strided
+ * axes are gathered FB lines at a time (FB consecutive z) for cache locality.
+ * work: 4*n*(FB+1) floats. */
+#define FB 16
 static void fft3d_plan(const fft_plan *p, float *d, int sign, float *work)
 {
     const int n = p->n;
     const size_t n2 = (size_t)n * n;
-    float *a = work, *b = work + 2 * n;
-    for (size_t l = 0; l < n2; l++) /* z lines */
-        {
-            memcpy(a, d + 2 * l * n, sizeof(float) * 2 * n);
-            fft_line(p, a, b, sign);
-            memcpy(d + 2 * l * n, a, sizeof(float) * 2 * n);
-        }
-    for (int x = 0; x < n; x++) /* y lines */
-        for (int z = 0; z < n; z++) {
-            float *base = d + 2 * ((size_t)x * n2 + z);
-            for (int y = 0; y < n; y++) { a[2 * y] = base[2 * (size_t)y * n]; a[2 * y + 1] = base[2 * (size_t)y * n + 1]; }
-            fft_line(p, a, b, sign);
-            for (int y = 0; y < n; y++) { base[2 * (size_t)y * n] = a[2 * y]; base[2 * (size_t)y * n + 1] = a[2 * y + 1]; }
-        }
-    for (size_t yz = 0; yz < n2; yz++) { /* x lines */
-        float *base = d + 2 * yz;
-        for (int x = 0; x < n; x++) { a[2 * x] = base[2 * (size_t)x * n2]; a[2 * x + 1] = base[2 * (size_t)x * n2 + 1]; }
+    float *a = work, *b = work + 2 * (size_t)n, *blk = work + 4 * (size_t)n; /* blk: FB lines */
+    for (size_t l = 0; l < n2; l++) { /* z lines, in place */
+        memcpy(a, d + 2 * l * n, sizeof(float) * 2 * n);
         fft_line(p, a, b, sign);
-        for (int x = 0; x < n; x++) { base[2 * (size_t)x * n2] = a[2 * x]; base[2 * (size_t)x * n2 + 1] = a[2 * x + 1]; }
+        memcpy(d + 2 * l * n, a, sizeof(float) * 2 * n);
+    }
+    for (int pass = 0; pass < 2; pass++) { /* pass 0: y lines, pass 1: x lines */
+        const size_t outer = pass == 0 ? (size_t)n : 1, inner = pass == 0 ? (size_t)n : n2;
+        const size_t stride = pass == 0 ? (size_t)n : n2; /* element stride along the axis */
+        for (size_t o = 0; o < outer; o++)
+            for (size_t z0 = 0; z0 < inner; z0 += FB) {
+                const int w = (int)((inner - z0) < FB ? (inner - z0) : FB);
+                float *base = d + 2 * (o * n2 + z0);
+                for (int e = 0; e < n; e++)
+                    for (int q = 0; q < w; q++) {
+                        blk[2 * ((size_t)q * n + e)] = base[2 * (e * stride + q)];
+                        blk[2 * ((size_t)q * n + e) + 1] = base[2 * (e * stride + q) + 1];
+                    }
+                for (int q = 0; q < w; q++) {
+                    memcpy(a, blk + 2 * (size_t)q * n, sizeof(float) * 2 * n);
+                    fft_line(p, a, b, sign);
+                    memcpy(blk + 2 * (size_t)q * n, a, sizeof(float) * 2 * n);
+                }
+                for (int e = 0; e < n; e++)
+                    for (int q = 0; q < w; q++) {
+                        base[2 * (e * stride + q)] = blk[2 * ((size_t)q * n + e)];
+                        base[2 * (e * stride + q) + 1] = blk[2 * ((size_t)q * n + e) + 1];
+                    }
+            }
     }
 }
 
@@ -523,7 +553,7 @@ int or_fft3d(int nf, const float *in, float *out, int sign)
     const size_t nf3 = (size_t)nf * nf * nf;
     if (out != in) memcpy(out, in, sizeof(float) * 2 * nf3);
     fft_plan *p = plan_create(nf);
-    float *work = (float *)malloc(sizeof(float) * 4 * nf);
+    float *work = (float *)malloc(sizeof(float) * 4 * (size_t)nf * (FB + 1));
     fft3d_plan(p, out, sign, work);
     free(work);
     plan_destroy(p);
@@ -627,7 +657,7 @@ int or_dock(int ng, int nf, float gw, int na, const float *coord, const float *c
         float *gr = (float *)malloc(sizeof(float) * 2 * ng3), *gi = gr + ng3;
         float *fin = (float *)malloc(sizeof(float) * 2 * nf3);
         float *fout = (float *)malloc(sizeof(float) * 2 * nf3);
-        float *work = (float *)malloc(sizeof(float) * 4 * (size_t)nf);
+        float *work = (float *)malloc(sizeof(float) * 4 * (size_t)nf * (FB + 1));
 #pragma omp for schedule(dynamic, 1)
         for (int q = 0; q < n_rot; q++) {
             float r[9];
