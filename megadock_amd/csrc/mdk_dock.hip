@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -27,7 +28,6 @@
 #include "mdk_internal.h"
 #include "../../include/mdk_hip.h"
 
-#define MDK_MAXT 16
 #define MDK_NSTAGE 7  // voxelize, fwd_z, fwd_y, x_corr, inv_y, inv_z_top, reduce
 
 namespace {
@@ -50,42 +50,6 @@ constexpr int WZ = 16;   // lines per workgroup, z passes
 constexpr int WY = 16;   // lines per workgroup, y passes
 constexpr int WX = 8;    // kz columns per workgroup in the x pass (plus 8 partner columns)
 constexpr int NT = 256;  // threads per workgroup
-
-struct Cand {
-    float key;
-    int idx;
-};
-
-// candidate order: key descending, then linear index ascending
-__device__ __forceinline__ bool better(float ka, int ia, float kb, int ib) {
-    return ka > kb || (ka == kb && ia < ib);
-}
-
-// register top-t list with static indexing (t <= MDK_MAXT)
-struct TopList {
-    float k[MDK_MAXT];
-    int i[MDK_MAXT];
-    __device__ void init(int t, float k0) {
-#pragma unroll
-        for (int q = 0; q < MDK_MAXT; q++) {
-            k[q] = k0;
-            i[q] = 0;
-        }
-    }
-    __device__ void push(int t, float kc, int ic) {
-#pragma unroll
-        for (int q = 0; q < MDK_MAXT; q++) {
-            if (q < t && better(kc, ic, k[q], i[q])) {
-                const float tk = k[q];
-                const int ti = i[q];
-                k[q] = kc;
-                i[q] = ic;
-                kc = tk;
-                ic = ti;
-            }
-        }
-    }
-};
 
 __device__ __forceinline__ void load_tw(float2 *tw_s, const float2 *tw_g, int ntw) {
     for (int q = threadIdx.x; q < ntw; q += blockDim.x) tw_s[q] = tw_g[q];
@@ -433,6 +397,8 @@ struct mdk_ctx {
     hipStream_t stream = nullptr;
     FftPlan plan{};
     float2 *d_tw = nullptr;
+    float2 *d_tw4 = nullptr;  // exp(-2 pi i m / nf): twiddles of the specialised pipeline
+    bool use_pipe = false;    // compile-time specialised kernels exist for nf
     float2 *d_rec = nullptr;
     bool have_rec = false, have_lig = false;
     // ligand
@@ -466,6 +432,28 @@ size_t lds_bytes(const mdk_ctx *c, int lines) {
 int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     hipStream_t s = c->stream;
+    if (c->use_pipe) {
+        PipeArgs a;
+        a.B = B;
+        a.ng = ng;
+        a.nd = nd;
+        a.H = c->H;
+        a.Hp = c->Hp;
+        a.t = c->t;
+        a.nf3f = (float)((size_t)nf * nf * nf);
+        a.lg = c->d_lg;
+        a.a1 = c->d_a1;
+        a.a2 = c->d_a2;
+        a.cc = c->d_cc;
+        a.rec = c->d_rec;
+        a.cand = c->d_cand;
+        a.smap = smap_dev;
+        a.tw4 = c->d_tw4;
+        a.stream = s;
+        a.ev = ev;
+        HIPCHK(pipe_run(nf, a));
+        return 0;
+    }
     const int lines_z = B * ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
@@ -578,6 +566,18 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         return bail(1);
     if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail("mdk_create: twiddle upload failed"));
+    {
+        std::vector<float2> tw4(nf);
+        for (int m = 0; m < nf; m++) {
+            const double a = -2.0 * M_PI * m / nf;
+            tw4[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        if (dmalloc(&c->d_tw4, nf) ||
+            hipMemcpy(c->d_tw4, tw4.data(), sizeof(float2) * nf, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail("mdk_create: twiddle upload failed"));
+        const char *g = std::getenv("MDK_FORCE_GENERIC");
+        c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
+    }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail("mdk_create: event creation failed"));
     // the largest kernel LDS request must fit
@@ -596,7 +596,7 @@ int mdk_destroy(mdk_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->d_tw, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
+    void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
                     c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
                     c->d_cc, c->d_cand, c->d_bout, c->d_results};
     for (void *p : ptrs)

@@ -25,3 +25,70 @@ hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, uint32
                            int *vidx, float2 *lg, hipStream_t s);
 
 inline int mask_words(int ng) { return (ng * ng * ng + 31) / 32; }
+
+#define MDK_MAXT 16  // largest t (results per rotation) the device top-t lists hold
+
+struct Cand {
+    float key;
+    int idx;
+};
+
+// candidate order: key descending, then linear index ascending
+__device__ __forceinline__ bool better(float ka, int ia, float kb, int ib) {
+    return ka > kb || (ka == kb && ia < ib);
+}
+
+// register top-t list with static indexing (t <= MDK_MAXT); (wk, wi) mirrors the
+// t-th entry so that most candidates are rejected with one comparison
+struct TopList {
+    float k[MDK_MAXT];
+    int i[MDK_MAXT];
+    float wk;
+    int wi;
+    __device__ void init(int t, float k0) {
+#pragma unroll
+        for (int q = 0; q < MDK_MAXT; q++) {
+            k[q] = k0;
+            i[q] = 0;
+        }
+        wk = k0;
+        wi = 0;
+    }
+    __device__ void push(int t, float kc, int ic) {
+        if (!better(kc, ic, wk, wi)) return;
+#pragma unroll
+        for (int q = 0; q < MDK_MAXT; q++) {
+            if (q < t && better(kc, ic, k[q], i[q])) {
+                const float tk = k[q];
+                const int ti = i[q];
+                k[q] = kc;
+                i[q] = ic;
+                kc = tk;
+                ic = ti;
+            }
+            if (q == t - 1) {
+                wk = k[q];
+                wi = i[q];
+            }
+        }
+    }
+};
+
+
+// Arguments of the FFT-side pipeline for one batch (buffers as in mdk_dock.hip).
+struct PipeArgs {
+    int B, ng, nd, H, Hp, t;
+    float nf3f;
+    const float2 *lg;
+    float2 *a1, *a2, *cc;
+    const float2 *rec;
+    Cand *cand;
+    float *smap;
+    const float2 *tw4;  // exp(-2 pi i m / nf), m < nf
+    hipStream_t stream;
+    hipEvent_t *ev;     // optional: 5 events, one after each pass
+};
+
+// Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.
+bool pipe_available(int nf);
+hipError_t pipe_run(int nf, const PipeArgs &a);

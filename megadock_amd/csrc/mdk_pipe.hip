@@ -1,0 +1,334 @@
+// FFT-side docking pipeline specialised at compile time for the common grid sizes.
+// Same data flow and buffers as the generic kernels in mdk_dock.hip (see there), but
+// every 1D transform is a four-step register FFT (mdk_regfft.h) with a single LDS
+// exchange, and the correlation pass reuses each receptor tile for RB rotations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mdk_internal.h"
+#include "mdk_regfft.h"
+
+namespace {
+
+using rf::Geo;
+using rf::sfor;
+
+constexpr int RB = 4;  // rotations per workgroup in the correlation pass
+
+// ------------------------------------------------------------------ forward z
+// lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1
+template <int N, int L>
+__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(const float2 *__restrict__ lg, float2 *__restrict__ a1,
+                                                          const float2 *__restrict__ twg, int ng, int nd, int nlines) {
+    constexpr int T = Geo<N>::T, LS = L + 1, NT = L * T;
+    __shared__ float2 S[N * LS];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
+    const int line0 = blockIdx.x * L;
+    for (int q = tid; q < L * N; q += NT) {
+        const int l = q / N, e = q % N;
+        float2 v = make_float2(0.f, 0.f);
+        if (line0 + l < nlines && e >= nd && e < nd + ng) v = lg[(size_t)(line0 + l) * ng + (e - nd)];
+        S[e * LS + l] = v;
+    }
+    __syncthreads();
+    rf::four_step_inplace<N, L, false>(S, tw, LS, line, u, true);
+    for (int q = tid; q < L * N; q += NT) {
+        const int l = q / N, e = q % N;
+        if (line0 + l < nlines) a1[(size_t)(line0 + l) * N + e] = S[e * LS + l];
+    }
+}
+
+// ------------------------------------------------------------------ forward y
+// block (kz chunk, x, b); rows y < ng of a1 sit at y + nd; loads straight to registers
+template <int N, int L>
+__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restrict__ a1, float2 *__restrict__ a2,
+                                                          const float2 *__restrict__ twg, int ng, int nd) {
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    __shared__ float2 S[N * LS];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
+    const int kz = blockIdx.x * L + line, x = blockIdx.y, b = blockIdx.z;
+    const bool col = kz < N;
+    const float2 *src = a1 + ((size_t)b * ng + x) * ng * N;
+    __syncthreads();
+    if (u < N2) {
+        float2 v[N1];
+        sfor<0, N1>([&](auto jc) {
+            constexpr int j1 = decltype(jc)::value;
+            const int y = u + N2 * j1 - nd;
+            v[j1] = (col && y >= 0 && y < ng) ? src[(size_t)y * N + kz] : make_float2(0.f, 0.f);
+        });
+        rf::step1_store<N, false>(v, S, tw, LS, line, u);
+    }
+    __syncthreads();
+    if (u < N1) {
+        float2 y[N2];
+        rf::step2_load<N, false>(y, S, LS, line, u);
+        float2 *dst = a2 + ((size_t)b * ng + x) * N * N;
+        if (col)
+            sfor<0, N2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                dst[(size_t)(u + N1 * k2) * N + kz] = y[k2];
+            });
+    }
+}
+
+// ------------------------------------------------------------------ x + correlation
+// block (kz chunk c, ky, rotation group).  Lines 0..L-1: (ky, kz = c L + l); lines
+// L..2L-1: partner columns (-ky, -kz).  Forward x-FFT of all 2L lines, the Hermitian
+// fold Q(k) = (conj(R(k)) L(k) + conj(conj(R(-k)) L(-k))) / 2, inverse x-FFT of Q.
+// The receptor values of the tile are loaded once and reused for RB rotations.
+template <int N, int L>
+__global__ void __launch_bounds__(2 * L *Geo<N>::T)
+    kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
+              const float2 *__restrict__ twg, int ng, int nd, int H, int Hp, int B) {
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
+    constexpr int L2 = 2 * L, LS2 = L2 + 1, LS = L + 1, NT = L2 * T;
+    constexpr int PER = (L * N + NT - 1) / NT;  // fold elements per thread
+    __shared__ float2 S[N * LS2];
+    __shared__ float2 Q[N * LS];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
+    const int c = blockIdx.x, ky = blockIdx.y, b0 = blockIdx.z * RB;
+    const int kyp = (N - ky) % N;
+    // receptor values of this thread's fold elements
+    float2 r1[PER], r2[PER];
+#pragma unroll
+    for (int p = 0; p < PER; p++) {
+        const int q = tid + p * NT;
+        r1[p] = r2[p] = make_float2(0.f, 0.f);
+        if (q < L * N) {
+            const int l = q % L, kx = q / L, kz = c * L + l;
+            if (kz < H) {
+                const int kxp = (N - kx) % N, kzp = (N - kz) % N;
+                r1[p] = rec[((size_t)kx * N + ky) * N + kz];
+                r2[p] = rec[((size_t)kxp * N + kyp) * N + kzp];
+            }
+        }
+    }
+    const int line = tid % L2, u = tid / L2;
+    const int kzl = c * L + (line % L);
+    const bool valid_line = kzl < H;
+    const int kyl = line < L ? ky : kyp;
+    const int kzs = line < L ? kzl : (N - kzl) % N;
+    for (int bb = 0; bb < RB; bb++) {
+        const int b = b0 + bb;
+        if (b >= B) break;  // uniform across the block
+        const float2 *src = a2 + (size_t)b * ng * N * N;
+        __syncthreads();  // S, Q free (previous rotation done)
+        if (u < N2) {
+            float2 v[N1];
+            sfor<0, N1>([&](auto jc) {
+                constexpr int j1 = decltype(jc)::value;
+                const int x = u + N2 * j1 - nd;
+                v[j1] = (valid_line && x >= 0 && x < ng) ? src[((size_t)x * N + kyl) * N + kzs] : make_float2(0.f, 0.f);
+            });
+            rf::step1_store<N, false>(v, S, tw, LS2, line, u);
+        }
+        __syncthreads();
+        float2 y[N2];
+        if (u < N1) rf::step2_load<N, false>(y, S, LS2, line, u);
+        __syncthreads();
+        if (u < N1)
+            sfor<0, N2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                S[(u + N1 * k2) * LS2 + line] = y[k2];
+            });
+        __syncthreads();
+        // Hermitian fold into Q
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int q = tid + p * NT;
+            if (q < L * N) {
+                const int l = q % L, kx = q / L, kxp = (N - kx) % N;
+                const float2 l1 = S[kx * LS2 + l], l2 = S[kxp * LS2 + L + l];
+                const float2 a = r1[p], d = r2[p];
+                const float2 p1 = make_float2(a.x * l1.x + a.y * l1.y, a.x * l1.y - a.y * l1.x);
+                const float2 p2 = make_float2(d.x * l2.x + d.y * l2.y, d.x * l2.y - d.y * l2.x);
+                Q[kx * LS + l] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
+            }
+        }
+        __syncthreads();
+        // inverse x-FFT of the L lines of Q; thread (lq, uq) over the first L*T threads
+        const int lq = tid % L, uq = tid / L;
+        const bool act = tid < L * T;
+        if (act && uq < N2) {
+            float2 v[N1];
+            sfor<0, N1>([&](auto jc) {
+                constexpr int j1 = decltype(jc)::value;
+                v[j1] = Q[(uq + N2 * j1) * LS + lq];
+            });
+            rf::step1_store<N, true>(v, S, tw, LS, lq, uq);
+        }
+        __syncthreads();
+        if (act && uq < N1) {
+            float2 z[N2];
+            rf::step2_load<N, true>(z, S, LS, lq, uq);
+            const int kz = c * L + lq;
+            if (kz < Hp) {
+                float2 *dst = cc + (size_t)b * N * N * Hp;
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    dst[((size_t)(uq + N1 * k2) * N + ky) * Hp + kz] = z[k2];
+                });
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ inverse y, in place
+template <int N, int L>
+__global__ void __launch_bounds__(L *Geo<N>::T) kp_inv_y(float2 *__restrict__ cc, const float2 *__restrict__ twg, int Hp) {
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    __shared__ float2 S[N * LS];
+    __shared__ float2 tw[N];
+    const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
+    const int kz = blockIdx.x * L + line, x = blockIdx.y, b = blockIdx.z;
+    float2 *base = cc + ((size_t)b * N + x) * N * Hp;
+    __syncthreads();
+    if (u < N2) {
+        float2 v[N1];
+        sfor<0, N1>([&](auto jc) {
+            constexpr int j1 = decltype(jc)::value;
+            v[j1] = base[(size_t)(u + N2 * j1) * Hp + kz];
+        });
+        rf::step1_store<N, true>(v, S, tw, LS, line, u);
+    }
+    __syncthreads();
+    if (u < N1) {
+        float2 y[N2];
+        rf::step2_load<N, true>(y, S, LS, line, u);
+        sfor<0, N2>([&](auto kc) {
+            constexpr int k2 = decltype(kc)::value;
+            base[(size_t)(u + N1 * k2) * Hp + kz] = y[k2];
+        });
+    }
+}
+
+// ------------------------------------------------------------------ inverse z + top-t
+template <int N, int L>
+__global__ void __launch_bounds__(L *Geo<N>::T)
+    kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
+                 const float2 *__restrict__ twg, int H, int Hp, int nrows, int t, float nf3f) {
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    __shared__ float2 S[N * LS];
+    __shared__ float2 tw[N];
+    extern __shared__ Cand lists[];  // [L][T][t]
+    const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
+    const int row0 = blockIdx.x * L;
+    for (int q = tid; q < L * H; q += NT) {
+        const int l = q / H, kz = q % H;
+        float2 v = make_float2(0.f, 0.f);
+        if (row0 + l < nrows) v = cc[(size_t)(row0 + l) * Hp + kz];
+        S[kz * LS + l] = v;
+        if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    {
+        float2 v[N1];
+        if (u < N2)
+            sfor<0, N1>([&](auto jc) {
+                constexpr int j1 = decltype(jc)::value;
+                v[j1] = S[(u + N2 * j1) * LS + line];
+            });
+        __syncthreads();  // step 1 writes back into S
+        if (u < N2) rf::step1_store<N, true>(v, S, tw, LS, line, u);
+    }
+    __syncthreads();
+    const int r = row0 + line;
+    TopList tl;
+    tl.init(t, -INFINITY);
+    if (u < N1) {
+        float2 y[N2];
+        rf::step2_load<N, true>(y, S, LS, line, u);
+        if (r < nrows) {
+            if (smap) {
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    smap[(size_t)r * N + u + N1 * k2] = y[k2].x;
+                });
+            } else {
+                const int rr = (r % (N * N)) * N;
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const float v = y[k2].x;
+                    tl.push(t, (t == 1) ? v : v / nf3f, rr + u + N1 * k2);
+                });
+            }
+        }
+    }
+    if (smap) return;
+    Cand *my = lists + ((size_t)line * T + u) * t;
+    for (int q = 0; q < t; q++) {
+#pragma unroll
+        for (int s = 0; s < MDK_MAXT; s++)
+            if (s == q) my[q] = Cand{tl.k[s], tl.i[s]};
+    }
+    __syncthreads();
+    if (u == 0 && r < nrows) {
+        TopList m;
+        m.init(t, -INFINITY);
+        const Cand *all = lists + (size_t)line * T * t;
+        for (int q = 0; q < T * t; q++) m.push(t, all[q].key, all[q].idx);
+        for (int q = 0; q < t; q++) {
+#pragma unroll
+            for (int s = 0; s < MDK_MAXT; s++)
+                if (s == q) cand[(size_t)r * t + q] = Cand{m.k[s], m.i[s]};
+        }
+    }
+}
+
+template <int N> hipError_t run_pipe(const PipeArgs &a) {
+    constexpr int T = Geo<N>::T;
+    constexpr int L = 16, LX = 8;
+    const int B = a.B, ng = a.ng, nd = a.nd;
+    hipStream_t s = a.stream;
+    const int lines_z = B * ng * ng;
+    kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.lg, a.a1, a.tw4, ng, nd, lines_z);
+    if (a.ev) (void)hipEventRecord(a.ev[0], s);
+    kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4, ng, nd);
+    if (a.ev) (void)hipEventRecord(a.ev[1], s);
+    kp_x_corr<N, LX><<<dim3(a.Hp / LX, N, (B + RB - 1) / RB), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng, nd,
+                                                                                 a.H, a.Hp, B);
+    if (a.ev) (void)hipEventRecord(a.ev[2], s);
+    kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
+    if (a.ev) (void)hipEventRecord(a.ev[3], s);
+    const int rows = B * N * N;
+    const size_t lds_lists = sizeof(Cand) * (size_t)L * T * a.t;
+    const size_t lds_total = lds_lists + sizeof(float2) * (size_t)N * (L + 2);
+    if (lds_total > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kp_inv_z_top<N, L>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lists);
+        if (e != hipSuccess) return e;
+    }
+    kp_inv_z_top<N, L><<<(rows + L - 1) / L, L * T, lds_lists, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, rows,
+                                                                   a.t, a.nf3f);
+    if (a.ev) (void)hipEventRecord(a.ev[4], s);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool pipe_available(int nf) {
+    switch (nf) {
+        case 80: case 96: case 144: case 256: return true;
+        default: return false;
+    }
+}
+
+hipError_t pipe_run(int nf, const PipeArgs &a) {
+    switch (nf) {
+        case 80: return run_pipe<80>(a);
+        case 96: return run_pipe<96>(a);
+        case 144: return run_pipe<144>(a);
+        case 256: return run_pipe<256>(a);
+        default: return hipErrorInvalidValue;
+    }
+}

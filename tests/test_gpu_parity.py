@@ -57,6 +57,26 @@ def _lig_ctx(lig, ng, t=1, max_batch=16):
     return ctx
 
 
+@pytest.fixture(params=["specialised", "generic"])
+def kernels(request):
+    """Run with the compile-time specialised pipeline (when built for the size) and with
+    the generic mixed-radix kernels (MDK_FORCE_GENERIC=1, read when a context is made)."""
+    old = os.environ.get("MDK_FORCE_GENERIC")
+    os.environ["MDK_FORCE_GENERIC"] = "1" if request.param == "generic" else "0"
+    yield request.param
+    if old is None:
+        os.environ.pop("MDK_FORCE_GENERIC", None)
+    else:
+        os.environ["MDK_FORCE_GENERIC"] = old
+
+
+@pytest.fixture(scope="module")
+def synth(golden_dir):
+    rec = O.Protein(os.path.join(golden_dir, "synth_r120.pdb"))
+    lig = O.Protein(os.path.join(golden_dir, "synth_l40.pdb"))
+    return rec, lig
+
+
 def test_voxelize_bit_exact_1gcq(gpu, gcq):
     rec, lig, ng = gcq
     _say("voxelize 1gcq ng=%d" % ng)
@@ -113,16 +133,21 @@ def test_receptor_spectrum_vs_fp64(gpu, gcq):
     ctx.close()
 
 
-def test_score_map_vs_fp64(gpu, gcq):
-    rec, lig, ng = gcq
+@pytest.mark.parametrize("which,ng", [("gcq", None), ("synth", 48), ("synth", 72), ("synth", 128), ("synth", 50)])
+def test_score_map_vs_fp64(gpu, gcq, synth, kernels, which, ng):
+    """nf = 80, 96, 144, 256 (specialised kernels exist) and 100 (generic only)."""
+    if which == "gcq":
+        rec, lig, ng = gcq
+    else:
+        rec, lig = synth
     nf = 2 * ng
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     ctx = _lig_ctx(lig, ng)
     ctx.set_receptor(gr, gi)
     ang = O.load_rotations(3600)
     remb = O.embed(ng, nf, gr, gi)
-    for a in (0, 1234, 3599):
-        _say("score map rotation %d" % a)
+    for a in ((0, 1234, 3599) if nf < 200 else (1234,)):
+        _say("score map nf=%d rotation %d (%s)" % (nf, a, kernels))
         smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
         xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
         lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
@@ -137,17 +162,20 @@ def _near_tie_ok(smap64, i_gpu, i_ref, tol):
     return abs(smap64[i_gpu] - smap64[i_ref]) <= tol * abs(smap64).max()
 
 
-@pytest.mark.parametrize("t", [1, 3])
-def test_dock_vs_oracle(gpu, gcq, t):
-    rec, lig, ng = gcq
+@pytest.mark.parametrize("t,which,ng", [(1, "gcq", None), (3, "gcq", None), (3, "synth", 72), (2, "synth", 48)])
+def test_dock_vs_oracle(gpu, gcq, synth, kernels, t, which, ng):
+    if which == "gcq":
+        rec, lig, ng = gcq
+    else:
+        rec, lig = synth
     nf = 2 * ng
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     rr, ri = O.receptor_fft(ng, nf, gr, gi)
     ctx = _lig_ctx(lig, ng, t=t)
     ctx.set_receptor(gr, gi)
     ang = O.load_rotations(3600)
-    ids = np.arange(0, 3600, 150, dtype=np.int32)
-    _say("dock %d rotations t=%d nf=%d" % (len(ids), t, nf))
+    ids = np.arange(0, 3600, 150 if nf < 120 else 450, dtype=np.int32)
+    _say("dock %d rotations t=%d nf=%d (%s)" % (len(ids), t, nf, kernels))
     got = np.concatenate([ctx.dock(M.rotation_coefs(ang[ids[q:q + 16]]), ids[q:q + 16])
                           for q in range(0, len(ids), 16)])
     ref = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, t, threads=8)
@@ -168,7 +196,7 @@ def test_dock_vs_oracle(gpu, gcq, t):
     ctx.close()
 
 
-def test_dock_range_matches_batches_and_topn(gpu, gcq):
+def test_dock_range_matches_batches_and_topn(gpu, gcq, kernels):
     rec, lig, ng = gcq
     nf = 2 * ng
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
