@@ -480,7 +480,8 @@ int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_s
     if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
     if (run_fft_pipeline(c, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
-    k_reduce_top<<<B, NT, 0, c->stream>>>(c->d_cand, out_dev, id_dev, nf * nf, c->t, nf, (float)((size_t)nf * nf * nf));
+    const int lists = c->use_pipe ? pipe_cands_per_rotation(nf) : nf * nf;
+    k_reduce_top<<<B, NT, 0, c->stream>>>(c->d_cand, out_dev, id_dev, lists, c->t, nf, (float)((size_t)nf * nf * nf));
     if (ev) HIPCHK(hipEventRecord(ev[7], c->stream));
     HIPCHK(hipGetLastError());
     return 0;
@@ -531,6 +532,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     if (nf < 4 || ng < 2 || ng > nf || na < 1 || t < 1 || t > MDK_MAXT || max_batch < 1)
         return fail("mdk_create: invalid sizes (need 4 <= nf, 2 <= ng <= nf, na >= 1, 1 <= t <= 16, max_batch >= 1)");
     if (nf != 2 * ng) return fail("mdk_create: nf must be 2*ng (src/docking.cpp:23-25)");
+    if (na > 65535) return fail("mdk_create: at most 65535 ligand atoms");
     mdk_ctx *c = new mdk_ctx();
     c->device = device;
     c->nf = nf;
@@ -559,7 +561,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     if (dmalloc(&c->d_tw, tw.size()) || dmalloc(&c->d_rec, nf3) || dmalloc(&c->d_xyz, 3 * (size_t)na) ||
         dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
         dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) ||
-        dmalloc(&c->d_bits, 2 * B * mask_words(ng)) || dmalloc(&c->d_vidx, B * na) ||
+        dmalloc(&c->d_bits, B * vox_scratch_words(ng)) || dmalloc(&c->d_vidx, B * na) ||
         dmalloc(&c->d_lg, B * ng3) || dmalloc(&c->d_a1, B * ng * ng * nf) ||
         dmalloc(&c->d_a2, B * ng * nf * nf) || dmalloc(&c->d_cc, B * nf * nf * c->Hp) ||
         dmalloc(&c->d_cand, B * nf * nf * t) || dmalloc(&c->d_bout, B * t))

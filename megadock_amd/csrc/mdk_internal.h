@@ -19,12 +19,16 @@ struct LigandDev {
 };
 
 // Ligand voxelization of B rotations into lg[b*ng^3 + v] = (grid_r, grid_i).
-// rot: B*9 rotation coefficients (row-major r11..r33).  core/surf: 2 * B * nw words of
-// bit masks (nw = ceil(ng^3 / 32)), vidx: B*na ints; all three are scratch.
+// rot: B*9 rotation coefficients (row-major r11..r33).  bits: B * vox_scratch_words(ng)
+// words (core and surface bit masks, per-voxel atom counts), vidx: B*na ints; scratch.
 hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, uint32_t *bits,
                            int *vidx, float2 *lg, hipStream_t s);
 
 inline int mask_words(int ng) { return (ng * ng * ng + 31) / 32; }
+// per-voxel atom counts, two 16-bit counters per word (ligands have < 65536 atoms)
+inline int count_words(int ng) { return (ng * ng * ng + 1) / 2; }
+// scratch words launch_voxelize needs per rotation (core bits, surface bits, counts)
+inline size_t vox_scratch_words(int ng) { return 2 * (size_t)mask_words(ng) + count_words(ng); }
 
 #define MDK_MAXT 16  // largest t (results per rotation) the device top-t lists hold
 
@@ -53,6 +57,20 @@ struct TopList {
         }
         wk = k0;
         wi = 0;
+    }
+    // insert another list sorted best-first; stops at its first rejected entry
+    __device__ void merge_sorted(int t, const Cand *other) {
+        for (int q = 0; q < t; q++) {
+            if (!better(other[q].key, other[q].idx, wk, wi)) break;
+            push(t, other[q].key, other[q].idx);
+        }
+    }
+    __device__ void store(int t, Cand *dst) const {
+        for (int q = 0; q < t; q++) {
+#pragma unroll
+            for (int s = 0; s < MDK_MAXT; s++)
+                if (s == q) dst[q] = Cand{k[s], i[s]};
+        }
     }
     __device__ void push(int t, float kc, int ic) {
         if (!better(kc, ic, wk, wi)) return;
@@ -91,4 +109,6 @@ struct PipeArgs {
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.
 bool pipe_available(int nf);
+// top-t candidate lists the specialised pipeline leaves per rotation (each of t entries)
+int pipe_cands_per_rotation(int nf);
 hipError_t pipe_run(int nf, const PipeArgs &a);

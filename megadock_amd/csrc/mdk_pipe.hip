@@ -212,21 +212,27 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_inv_y(float2 *__restrict__ cc
 }
 
 // ------------------------------------------------------------------ inverse z + top-t
+// block (row chunk, b): rows y0..y0+L-1 of rotation b, row r = x*N + y; the row's
+// spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps the top-t
+// of its N2 outputs; a tree merge over the block leaves the block's top-t, written as
+// t candidates (or, for the test hook, the real score rows go to smap).
 template <int N, int L>
 __global__ void __launch_bounds__(L *Geo<N>::T)
     kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
-                 const float2 *__restrict__ twg, int H, int Hp, int nrows, int t, float nf3f) {
+                 const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    constexpr int NROW = N * N;
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
-    extern __shared__ Cand lists[];  // [L][T][t]
+    extern __shared__ Cand lists[];  // [NT][t]
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int row0 = blockIdx.x * L;
+    const int b = blockIdx.y, row0 = blockIdx.x * L;
+    const float2 *src = cc + ((size_t)b * NROW) * Hp;
     for (int q = tid; q < L * H; q += NT) {
         const int l = q / H, kz = q % H;
         float2 v = make_float2(0.f, 0.f);
-        if (row0 + l < nrows) v = cc[(size_t)(row0 + l) * Hp + kz];
+        if (row0 + l < NROW) v = src[(size_t)(row0 + l) * Hp + kz];
         S[kz * LS + l] = v;
         if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
     }
@@ -248,41 +254,31 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
     if (u < N1) {
         float2 y[N2];
         rf::step2_load<N, true>(y, S, LS, line, u);
-        if (r < nrows) {
+        if (r < NROW) {
             if (smap) {
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
-                    smap[(size_t)r * N + u + N1 * k2] = y[k2].x;
+                    smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
                 });
             } else {
-                const int rr = (r % (N * N)) * N;
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
                     const float v = y[k2].x;
-                    tl.push(t, (t == 1) ? v : v / nf3f, rr + u + N1 * k2);
+                    tl.push(t, (t == 1) ? v : v / nf3f, r * N + u + N1 * k2);
                 });
             }
         }
     }
-    if (smap) return;
-    Cand *my = lists + ((size_t)line * T + u) * t;
-    for (int q = 0; q < t; q++) {
-#pragma unroll
-        for (int s = 0; s < MDK_MAXT; s++)
-            if (s == q) my[q] = Cand{tl.k[s], tl.i[s]};
+    if (smap) return;  // uniform
+    // tree merge of the NT per-thread lists
+    constexpr int P2 = (NT & (NT - 1)) == 0 ? NT : 1 << (32 - __builtin_clz(NT));
+    for (int s = P2 / 2; s >= 1; s >>= 1) {
+        if (tid >= s && tid < 2 * s && tid < NT) tl.store(t, lists + (size_t)(tid - s) * t);
+        __syncthreads();
+        if (tid < s && tid + s < NT) tl.merge_sorted(t, lists + (size_t)tid * t);
+        __syncthreads();
     }
-    __syncthreads();
-    if (u == 0 && r < nrows) {
-        TopList m;
-        m.init(t, -INFINITY);
-        const Cand *all = lists + (size_t)line * T * t;
-        for (int q = 0; q < T * t; q++) m.push(t, all[q].key, all[q].idx);
-        for (int q = 0; q < t; q++) {
-#pragma unroll
-            for (int s = 0; s < MDK_MAXT; s++)
-                if (s == q) cand[(size_t)r * t + q] = Cand{m.k[s], m.i[s]};
-        }
-    }
+    if (tid == 0) tl.store(t, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
 template <int N> hipError_t run_pipe(const PipeArgs &a) {
@@ -300,7 +296,6 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
     if (a.ev) (void)hipEventRecord(a.ev[3], s);
-    const int rows = B * N * N;
     const size_t lds_lists = sizeof(Cand) * (size_t)L * T * a.t;
     const size_t lds_total = lds_lists + sizeof(float2) * (size_t)N * (L + 2);
     if (lds_total > 64 * 1024) {
@@ -308,13 +303,23 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lists);
         if (e != hipSuccess) return e;
     }
-    kp_inv_z_top<N, L><<<(rows + L - 1) / L, L * T, lds_lists, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, rows,
-                                                                   a.t, a.nf3f);
+    kp_inv_z_top<N, L><<<dim3((N * N + L - 1) / L, B), L * T, lds_lists, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H,
+                                                                             a.Hp, a.t, a.nf3f);
     if (a.ev) (void)hipEventRecord(a.ev[4], s);
     return hipGetLastError();
 }
 
 }  // namespace
+
+int pipe_cands_per_rotation(int nf) {
+    switch (nf) {
+        case 80: return (80 * 80 + 15) / 16;
+        case 96: return (96 * 96 + 15) / 16;
+        case 144: return (144 * 144 + 15) / 16;
+        case 256: return (256 * 256 + 15) / 16;
+        default: return 0;
+    }
+}
 
 bool pipe_available(int nf) {
     switch (nf) {

@@ -36,9 +36,24 @@ __global__ void k_clear(uint32_t *p, size_t n) {
         p[i] = 0u;
 }
 
-// one thread per (atom, rotation): core/surface bits and the charge voxel
+// set bits [k0, k1) of the row starting at voxel v0 (k1 - k0 <= 32)
+__device__ __forceinline__ void or_run(uint32_t *m, int v0, int k0, int k1) {
+    if (k1 <= k0) return;
+    const int a = v0 + k0, e = v0 + k1;  // [a, e)
+    const int w0 = a >> 5, w1 = (e - 1) >> 5;
+    if (w0 == w1) {
+        const uint32_t n = (uint32_t)(e - a);
+        const uint32_t mask = (n == 32 ? 0xffffffffu : ((1u << n) - 1u)) << (a & 31);
+        atomicOr(&m[w0], mask);
+    } else {
+        atomicOr(&m[w0], 0xffffffffu << (a & 31));
+        atomicOr(&m[w1], 0xffffffffu >> (31 - ((e - 1) & 31)));
+    }
+}
+
+// one thread per (atom, rotation): core/surface bits, the charge voxel and its atom count
 __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t *core,
-                            uint32_t *surf, int *vidx, int nw) {
+                            uint32_t *surf, uint32_t *cnt, int *vidx, int nw, int cw) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (l >= L.na) return;
@@ -65,19 +80,26 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t
             const float dy = Y - L.gc[j];
             const float d2 = xd + dy * dy;
             if (d2 > rc2) continue;
+            // the hits along k form one run per sphere (d3 grows away from the atom), but
+            // each voxel is tested exactly as the reference does
+            int c0 = kb, c1 = ka, s0 = kb, s1 = ka;
             for (int k = ka; k < kb; k++) {
                 const float dz = Z - L.gc[k];
                 const float d3 = d2 + dz * dz;
-                const int v = (i * ng + j) * ng + k;
-                if (d3 < rc2) atomicOr(&cb[v >> 5], 1u << (v & 31));
-                if (d3 < rs2) atomicOr(&sb[v >> 5], 1u << (v & 31));
+                if (d3 < rc2) { c0 = min(c0, k); c1 = k + 1; }
+                if (d3 < rs2) { s0 = min(s0, k); s1 = k + 1; }
             }
+            const int v0 = (i * ng + j) * ng;
+            or_run(cb, v0, c0, c1);
+            or_run(sb, v0, s0, s1);
         }
     }
     const int ci = max(0, min(ng - 1, (int)((X + L.pad) / L.gw)));
     const int cj = max(0, min(ng - 1, (int)((Y + L.pad) / L.gw)));
     const int ck = max(0, min(ng - 1, (int)((Z + L.pad) / L.gw)));
-    vidx[(size_t)b * L.na + l] = (ci * ng + cj) * ng + ck;
+    const int v = (ci * ng + cj) * ng + ck;
+    vidx[(size_t)b * L.na + l] = v;
+    atomicAdd(&cnt[(size_t)b * cw + (v >> 1)], 1u << (16 * (v & 1)));
 }
 
 __device__ __forceinline__ bool bit(const uint32_t *m, int v) { return (m[v >> 5] >> (v & 31)) & 1u; }
@@ -119,34 +141,43 @@ __global__ void k_vox_grid(const uint32_t *core, const uint32_t *surf, float2 *l
     lg[(size_t)b * ng3 + v] = make_float2(val, 0.0f);
 }
 
-// The first atom (lowest index) of each charge voxel sums that voxel's charges in
-// atom order, starting from +0.0f, and is its only writer.
-__global__ void k_vox_charge(const int *__restrict__ vidx, const float *__restrict__ charge,
-                             float2 *lg, int na, int ng3) {
+// Charges of the atoms nearest to each voxel, summed in atom order from +0.0f
+// (src/ligand.cpp:216-234).  With one or two atoms in a voxel the float sum does not
+// depend on the order (0 + a = a and a + b = b + a exactly), so those are added
+// atomically; a voxel holding three or more is summed in atom order by its
+// lowest-index atom, which is then its only writer.
+__global__ void k_vox_charge(const int *__restrict__ vidx, const uint32_t *__restrict__ cnt,
+                             const float *__restrict__ charge, float2 *lg, int na, int ng3, int cw) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (l >= na) return;
     const int *vi = vidx + (size_t)b * na;
     const int v = vi[l];
+    const uint32_t n = (cnt[(size_t)b * cw + (v >> 1)] >> (16 * (v & 1))) & 0xffffu;
+    float *dst = &lg[(size_t)b * ng3 + v].y;
+    if (n <= 2) {
+        atomicAdd(dst, charge[l]);
+        return;
+    }
     for (int q = 0; q < l; q++)
         if (vi[q] == v) return;
     float sum = 0.0f;
     for (int q = l; q < na; q++)
         if (vi[q] == v) sum += charge[q];
-    lg[(size_t)b * ng3 + v].y = sum;
+    *dst = sum;
 }
 
 }  // namespace
 
 hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, uint32_t *bits, int *vidx,
                            float2 *lg, hipStream_t s) {
-    const int nw = mask_words(lig.ng);
-    const size_t nbits = 2 * (size_t)B * nw;
-    uint32_t *core = bits, *surf = bits + (size_t)B * nw;
+    const int nw = mask_words(lig.ng), cw = count_words(lig.ng);
+    const size_t nclear = (size_t)B * (2 * (size_t)nw + cw);
+    uint32_t *core = bits, *surf = bits + (size_t)B * nw, *cnt = bits + 2 * (size_t)B * nw;
     const int ng3 = lig.ng * lig.ng * lig.ng;
-    k_clear<<<dim3((unsigned)((nbits + 255) / 256 < 65535 ? (nbits + 255) / 256 : 65535)), 256, 0, s>>>(bits, nbits);
-    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, core, surf, vidx, nw);
+    k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(bits, nclear);
+    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, core, surf, cnt, vidx, nw, cw);
     k_vox_grid<<<dim3((ng3 + 255) / 256, B), 256, 0, s>>>(core, surf, lg, lig.ng, nw, lig.delta);
-    k_vox_charge<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(vidx, lig.charge, lg, lig.na, ng3);
+    k_vox_charge<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(vidx, cnt, lig.charge, lg, lig.na, ng3, cw);
     return hipGetLastError();
 }
