@@ -93,6 +93,47 @@ struct TopList {
 };
 
 
+// top-TT list with a compile-time length: key descending, index ascending
+template <int TT> struct TopT {
+    float k[TT];
+    int i[TT];
+    __device__ __forceinline__ void init(float k0) {
+#pragma unroll
+        for (int q = 0; q < TT; q++) {
+            k[q] = k0;
+            i[q] = 0;
+        }
+    }
+    __device__ __forceinline__ void push(float kc, int ic) {
+        if (!better(kc, ic, k[TT - 1], i[TT - 1])) return;
+#pragma unroll
+        for (int q = 0; q < TT; q++) {
+            if (better(kc, ic, k[q], i[q])) {
+                const float tk = k[q];
+                const int ti = i[q];
+                k[q] = kc;
+                i[q] = ic;
+                kc = tk;
+                ic = ti;
+            }
+        }
+    }
+    // merge with the list of lane (lane ^ mask) of the same wavefront; partner_ok must be
+    // false when that lane does not exist (a partial last wavefront of the block)
+    __device__ __forceinline__ void merge_xor(int mask, bool partner_ok) {
+        float ok[TT];
+        int oi[TT];
+#pragma unroll
+        for (int q = 0; q < TT; q++) {
+            ok[q] = __shfl_xor(k[q], mask);
+            oi[q] = __shfl_xor(i[q], mask);
+        }
+        if (partner_ok)
+#pragma unroll
+            for (int q = 0; q < TT; q++) push(ok[q], oi[q]);
+    }
+};
+
 // Arguments of the FFT-side pipeline for one batch (buffers as in mdk_dock.hip).
 struct PipeArgs {
     int B, ng, nd, H, Hp, t;

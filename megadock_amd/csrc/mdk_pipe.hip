@@ -213,24 +213,25 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_inv_y(float2 *__restrict__ cc
 
 // ------------------------------------------------------------------ inverse z + top-t
 // block (row chunk, b): rows y0..y0+L-1 of rotation b, row r = x*N + y; the row's
-// spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps the top-t
-// of its N2 outputs; a tree merge over the block leaves the block's top-t, written as
-// t candidates (or, for the test hook, the real score rows go to smap).
-template <int N, int L>
+// spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps the top-TT
+// (TT >= t, compile time) of its N2 outputs; lists are merged across each wavefront with
+// lane shuffles and across wavefronts through LDS, leaving the block's top-t as t
+// candidates.  The test hook writes the real score rows to smap instead.
+template <int N, int L, int TT>
 __global__ void __launch_bounds__(L *Geo<N>::T)
     kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                  const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    constexpr int NROW = N * N;
+    constexpr int NROW = N * N, NW = (NT + 63) / 64;
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
-    extern __shared__ Cand lists[];  // [NT][t]
+    __shared__ Cand wl[NW * TT];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
     const int b = blockIdx.y, row0 = blockIdx.x * L;
     const float2 *src = cc + ((size_t)b * NROW) * Hp;
     for (int q = tid; q < L * H; q += NT) {
-        const int l = q / H, kz = q % H;
+        const int l = q / H, kz = q - l * H;
         float2 v = make_float2(0.f, 0.f);
         if (row0 + l < NROW) v = src[(size_t)(row0 + l) * Hp + kz];
         S[kz * LS + l] = v;
@@ -249,8 +250,8 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
     }
     __syncthreads();
     const int r = row0 + line;
-    TopList tl;
-    tl.init(t, -INFINITY);
+    TopT<TT> tl;
+    tl.init(-INFINITY);
     if (u < N1) {
         float2 y[N2];
         rf::step2_load<N, true>(y, S, LS, line, u);
@@ -261,24 +262,37 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
                     smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
                 });
             } else {
+                const bool raw = (t == 1);
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
                     const float v = y[k2].x;
-                    tl.push(t, (t == 1) ? v : v / nf3f, r * N + u + N1 * k2);
+                    tl.push(raw ? v : v / nf3f, r * N + u + N1 * k2);
                 });
             }
         }
     }
     if (smap) return;  // uniform
-    // tree merge of the NT per-thread lists
-    constexpr int P2 = (NT & (NT - 1)) == 0 ? NT : 1 << (32 - __builtin_clz(NT));
-    for (int s = P2 / 2; s >= 1; s >>= 1) {
-        if (tid >= s && tid < 2 * s && tid < NT) tl.store(t, lists + (size_t)(tid - s) * t);
-        __syncthreads();
-        if (tid < s && tid + s < NT) tl.merge_sorted(t, lists + (size_t)tid * t);
-        __syncthreads();
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) tl.merge_xor(m, (tid ^ m) < NT);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < TT; q++) wl[(tid >> 6) * TT + q] = Cand{tl.k[q], tl.i[q]};
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < NW; w++)
+#pragma unroll
+            for (int q = 0; q < TT; q++) tl.push(wl[w * TT + q].key, wl[w * TT + q].idx);
+        Cand *dst = cand + ((size_t)b * gridDim.x + blockIdx.x) * t;
+#pragma unroll
+        for (int q = 0; q < TT; q++)
+            if (q < t) dst[q] = Cand{tl.k[q], tl.i[q]};
     }
-    if (tid == 0) tl.store(t, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
+}
+
+template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
+    constexpr int T = Geo<N>::T;
+    kp_inv_z_top<N, L, TT><<<dim3((N * N + L - 1) / L, a.B), L * T, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H,
+                                                                           a.Hp, a.t, a.nf3f);
 }
 
 template <int N> hipError_t run_pipe(const PipeArgs &a) {
@@ -296,15 +310,11 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
     if (a.ev) (void)hipEventRecord(a.ev[3], s);
-    const size_t lds_lists = sizeof(Cand) * (size_t)L * T * a.t;
-    const size_t lds_total = lds_lists + sizeof(float2) * (size_t)N * (L + 2);
-    if (lds_total > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kp_inv_z_top<N, L>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lists);
-        if (e != hipSuccess) return e;
-    }
-    kp_inv_z_top<N, L><<<dim3((N * N + L - 1) / L, B), L * T, lds_lists, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H,
-                                                                             a.Hp, a.t, a.nf3f);
+    if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
+    else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
+    else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
+    else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
+    else launch_inv_z<N, L, 16>(a, s);
     if (a.ev) (void)hipEventRecord(a.ev[4], s);
     return hipGetLastError();
 }
