@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--cpu-rotations", type=int, default=0, help="CPU baseline sample size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
+    ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants and batch "
+                    "sizes, e.g. 'v0,v1,v3,b32,b128'; prints one stage-timing line per setting and exits")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -96,6 +98,23 @@ def main():
     ctx.set_rotations(rot)
     log("context ready: batch %d" % B)
 
+    if args.sweep:
+        for item in args.sweep.split(","):
+            os.environ["MDK_XCORR_VARIANT"] = item[1:] if item[0] == "v" else os.environ.get("MDK_XCORR_VARIANT", "0")
+            bs = int(item[1:]) if item[0] == "b" else B
+            sctx = M.Context(nf, ng, lig.na, t=t, max_batch=bs)
+            sctx.set_receptor(gr, gi)
+            sctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, M.grid_coord(ng), 1.2)
+            sctx.set_rotations(rot[:max(4 * bs, 2048)])
+            sctx.dock_range(0, 2 * bs)
+            sctx.sync()
+            ms = sctx.time_range(0, 16 * bs if 16 * bs <= max(4 * bs, 2048) else max(4 * bs, 2048), reps=2)
+            n = 16 * bs if 16 * bs <= max(4 * bs, 2048) else max(4 * bs, 2048)
+            st = sctx.profile_stages(0, bs)
+            print(json.dumps({"setting": item, "batch": bs, "rotations_per_s": round(n / (ms * 1e-3), 1),
+                              "stages_ms": {k: round(v, 4) for k, v in st.items()}}), flush=True)
+            sctx.close()
+        return
     for w in range(args.warmup):
         ctx.dock_range(0, R)
         ctx.sync()

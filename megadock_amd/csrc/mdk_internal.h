@@ -146,6 +146,7 @@ struct PipeArgs {
     const float2 *tw4;  // exp(-2 pi i m / nf), m < nf
     hipStream_t stream;
     hipEvent_t *ev;     // optional: 5 events, one after each pass
+    int variant;        // kernel variant of the correlation pass (MDK_XCORR_VARIANT)
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.

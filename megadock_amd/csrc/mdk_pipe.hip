@@ -14,7 +14,6 @@ namespace {
 using rf::Geo;
 using rf::sfor;
 
-constexpr int RB = 4;  // rotations per workgroup in the correlation pass
 
 // ------------------------------------------------------------------ forward z
 // lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1
@@ -82,7 +81,9 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
 // L..2L-1: partner columns (-ky, -kz).  Forward x-FFT of all 2L lines, the Hermitian
 // fold Q(k) = (conj(R(k)) L(k) + conj(conj(R(-k)) L(-k))) / 2, inverse x-FFT of Q.
 // The receptor values of the tile are loaded once and reused for RB rotations.
-template <int N, int L>
+// B_FAST: the rotation is the fastest grid index, so the workgroups that share a
+// receptor tile run back to back and find it in L2 / the last-level cache.
+template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
               const float2 *__restrict__ twg, int ng, int nd, int H, int Hp, int B) {
@@ -94,28 +95,33 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     __shared__ float2 tw[N];
     const int tid = threadIdx.x;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int c = blockIdx.x, ky = blockIdx.y, b0 = blockIdx.z * RB;
+    const int c = B_FAST ? blockIdx.y : blockIdx.x, ky = B_FAST ? blockIdx.z : blockIdx.y;
+    const int b0 = (B_FAST ? blockIdx.x : blockIdx.z) * RB;
     const int kyp = (N - ky) % N;
-    // receptor values of this thread's fold elements
-    float2 r1[PER], r2[PER];
-#pragma unroll
-    for (int p = 0; p < PER; p++) {
+    // receptor values of this thread's fold elements (kept in registers when CACHE_R)
+    auto rec_pair = [&](int p, float2 &a, float2 &d) {
         const int q = tid + p * NT;
-        r1[p] = r2[p] = make_float2(0.f, 0.f);
+        a = d = make_float2(0.f, 0.f);
         if (q < L * N) {
             const int l = q % L, kx = q / L, kz = c * L + l;
             if (kz < H) {
                 const int kxp = (N - kx) % N, kzp = (N - kz) % N;
-                r1[p] = rec[((size_t)kx * N + ky) * N + kz];
-                r2[p] = rec[((size_t)kxp * N + kyp) * N + kzp];
+                a = rec[((size_t)kx * N + ky) * N + kz];
+                d = rec[((size_t)kxp * N + kyp) * N + kzp];
             }
         }
+    };
+    float2 r1[CACHE_R ? PER : 1], r2[CACHE_R ? PER : 1];
+    if constexpr (CACHE_R) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) rec_pair(p, r1[p], r2[p]);
     }
     const int line = tid % L2, u = tid / L2;
     const int kzl = c * L + (line % L);
     const bool valid_line = kzl < H;
     const int kyl = line < L ? ky : kyp;
     const int kzs = line < L ? kzl : (N - kzl) % N;
+#pragma unroll 1
     for (int bb = 0; bb < RB; bb++) {
         const int b = b0 + bb;
         if (b >= B) break;  // uniform across the block
@@ -147,7 +153,13 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
             if (q < L * N) {
                 const int l = q % L, kx = q / L, kxp = (N - kx) % N;
                 const float2 l1 = S[kx * LS2 + l], l2 = S[kxp * LS2 + L + l];
-                const float2 a = r1[p], d = r2[p];
+                float2 a, d;
+                if constexpr (CACHE_R) {
+                    a = r1[p];
+                    d = r2[p];
+                } else {
+                    rec_pair(p, a, d);
+                }
                 const float2 p1 = make_float2(a.x * l1.x + a.y * l1.y, a.x * l1.y - a.y * l1.x);
                 const float2 p2 = make_float2(d.x * l2.x + d.y * l2.y, d.x * l2.y - d.y * l2.x);
                 Q[kx * LS + l] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
@@ -295,6 +307,171 @@ template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t
                                                                            a.Hp, a.t, a.nf3f);
 }
 
+// the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS
+constexpr int plane_stride(int n) { return ((n / 2 + 1) % 2) ? n / 2 + 1 : n / 2 + 2; }
+constexpr bool fused_inverse(int n) { return (size_t)n * plane_stride(n) * 8 <= 150 * 1024; }
+
+// ------------------------------------------------------------------ fused inverse y + z + top-t
+// One workgroup per (x, b) plane of cc (nf rows ky, kz <= nf/2 columns) held in LDS:
+//   y-IFFT of every kz column in place;
+//   per row, the half-length complex-to-real step: with M = nf/2 and w = exp(+2 pi i / nf),
+//     Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])),  k < M,
+//   whose M-point inverse gives z[m] = x[2m] + i x[2m+1] (x the real score row);
+//   the M-point inverse of each row, whose outputs go straight into the top-t lists.
+// The plane's top-t is written as one candidate list per (x, b).
+template <int N, int TT>
+__global__ void __launch_bounds__(1024)
+    kp_inv_yz_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
+                  const float2 *__restrict__ twg, int Hp, int t, float nf3f) {
+    constexpr int NTH = 1024, NW = NTH / 64;
+    constexpr int M = N / 2, H = M + 1;
+    constexpr int LSP = plane_stride(N);  // row stride of the plane (odd)
+    constexpr int N1y = Geo<N>::N1, N2y = Geo<N>::N2;
+    constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2;
+    constexpr int KY = (H * N1y + NTH - 1) / NTH;  // y tasks per thread
+    constexpr int KZ = (N * N1z + NTH - 1) / NTH;  // z tasks per thread
+    __shared__ float2 P[N * LSP];
+    __shared__ float2 tw[N];
+    __shared__ Cand wl[NW * TT];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < N; q += NTH) tw[q] = twg[q];
+    const int x = blockIdx.x, b = blockIdx.y;
+    const float2 *src = cc + ((size_t)b * N + x) * N * Hp;
+    for (int q = tid; q < N * H; q += NTH) {
+        const int ky = q / H, kz = q - ky * H;
+        P[ky * LSP + kz] = src[(size_t)ky * Hp + kz];
+    }
+    __syncthreads();
+    // ---- y-IFFT of the H columns (line = kz, element = ky): element stride LSP
+    {
+        float2 v[KY][N1y];
+#pragma unroll
+        for (int j = 0; j < KY; j++) {
+            const int task = tid + j * NTH, line = task % H, u = task / H;
+            if (u < N2y)
+                sfor<0, N1y>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    v[j][j1] = P[(u + N2y * j1) * LSP + line];
+                });
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KY; j++) {
+            const int task = tid + j * NTH, line = task % H, u = task / H;
+            if (u < N2y) rf::step1_g<N, true>(v[j], P, tw, 1, LSP, 1, line, u);
+        }
+        __syncthreads();
+        float2 y[KY][N2y];
+#pragma unroll
+        for (int j = 0; j < KY; j++) {
+            const int task = tid + j * NTH, line = task % H, u = task / H;
+            if (u < N1y) rf::step2_g<N, true>(y[j], P, LSP, 1, line, u);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KY; j++) {
+            const int task = tid + j * NTH, line = task % H, u = task / H;
+            if (u < N1y)
+                sfor<0, N2y>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    P[(u + N1y * k2) * LSP + line] = y[j][k2];
+                });
+        }
+    }
+    __syncthreads();
+    // ---- complex-to-real pre-step on each row, pairs (k, M-k) for k <= M/2
+    for (int q = tid; q < N * (M / 2 + 1); q += NTH) {
+        const int row = q / (M / 2 + 1), k = q - row * (M / 2 + 1);
+        float2 *R = P + row * LSP;
+        const float2 a = R[k], c = R[M - k];
+        // Z[k] from (X[k], X[M-k]); w^k = conj(tw[k])
+        const float2 wk = make_float2(tw[k].x, -tw[k].y);
+        const float2 s1 = make_float2(a.x + c.x, a.y - c.y);     // X[k] + conj(X[M-k])
+        const float2 d1 = make_float2(a.x - c.x, a.y + c.y);     // X[k] - conj(X[M-k])
+        const float2 e1 = make_float2(wk.x * d1.x - wk.y * d1.y, wk.x * d1.y + wk.y * d1.x);
+        const float2 z1 = make_float2(s1.x - e1.y, s1.y + e1.x);  // s1 + i e1
+        if (k == 0) {
+            R[0] = z1;
+        } else {
+            const int k2 = M - k;  // Z[M-k] from (X[M-k], X[k]); w^(M-k) = conj(tw[M-k])
+            const float2 w2 = make_float2(tw[k2].x, -tw[k2].y);
+            const float2 s2 = make_float2(c.x + a.x, c.y - a.y);
+            const float2 d2 = make_float2(c.x - a.x, c.y + a.y);
+            const float2 e2 = make_float2(w2.x * d2.x - w2.y * d2.y, w2.x * d2.y + w2.y * d2.x);
+            R[k] = z1;
+            R[k2] = make_float2(s2.x - e2.y, s2.y + e2.x);
+        }
+    }
+    __syncthreads();
+    // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
+    TopT<TT> tl;
+    tl.init(-INFINITY);
+    {
+        float2 v[KZ][N1z];
+#pragma unroll
+        for (int j = 0; j < KZ; j++) {
+            const int task = tid + j * NTH, line = task % N, u = task / N;
+            if (u < N2z)
+                sfor<0, N1z>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    v[j][j1] = P[line * LSP + u + N2z * j1];
+                });
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < KZ; j++) {
+            const int task = tid + j * NTH, line = task % N, u = task / N;
+            if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
+        }
+        __syncthreads();
+        const bool raw = (t == 1);
+#pragma unroll
+        for (int j = 0; j < KZ; j++) {
+            const int task = tid + j * NTH, line = task % N, u = task / N;
+            if (u < N1z) {
+                float2 y[N2z];
+                rf::step2_g<M, true>(y, P, 1, LSP, line, u);
+                const int base = (x * N + line) * N;
+                if (smap) {
+                    sfor<0, N2z>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const int m = u + N1z * k2;
+                        smap[(size_t)b * N * N * N + base + 2 * m] = y[k2].x;
+                        smap[(size_t)b * N * N * N + base + 2 * m + 1] = y[k2].y;
+                    });
+                } else {
+                    sfor<0, N2z>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const int m = u + N1z * k2;
+                        tl.push(raw ? y[k2].x : y[k2].x / nf3f, base + 2 * m);
+                        tl.push(raw ? y[k2].y : y[k2].y / nf3f, base + 2 * m + 1);
+                    });
+                }
+            }
+        }
+    }
+    if (smap) return;  // uniform
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) tl.merge_xor(m, true);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < TT; q++) wl[(tid >> 6) * TT + q] = Cand{tl.k[q], tl.i[q]};
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < NW; w++)
+#pragma unroll
+            for (int q = 0; q < TT; q++) tl.push(wl[w * TT + q].key, wl[w * TT + q].idx);
+        Cand *dst = cand + ((size_t)b * N + x) * t;
+#pragma unroll
+        for (int q = 0; q < TT; q++)
+            if (q < t) dst[q] = Cand{tl.k[q], tl.i[q]};
+    }
+}
+
+template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
+    kp_inv_yz_top<N, TT><<<dim3(N, a.B), 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f);
+}
+
 template <int N> hipError_t run_pipe(const PipeArgs &a) {
     constexpr int T = Geo<N>::T;
     constexpr int L = 16, LX = 8;
@@ -305,16 +482,42 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[0], s);
     kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4, ng, nd);
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
-    kp_x_corr<N, LX><<<dim3(a.Hp / LX, N, (B + RB - 1) / RB), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng, nd,
-                                                                                 a.H, a.Hp, B);
+    // correlation-pass variant: receptor tile cached in registers for 4 rotations (0),
+    // re-read from L2 for each of 4 rotations (1), one rotation per workgroup (2)
+    switch (a.variant) {
+        case 1:
+            kp_x_corr<N, LX, 4, false><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
+            break;
+        case 2:
+            kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng, nd,
+                                                                                   a.H, a.Hp, B);
+            break;
+        case 3:
+            kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng,
+                                                                                        nd, a.H, a.Hp, B);
+            break;
+        default:
+            kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
+    }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
-    kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
-    if (a.ev) (void)hipEventRecord(a.ev[3], s);
-    if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
-    else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
-    else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
-    else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
-    else launch_inv_z<N, L, 16>(a, s);
+    if constexpr (fused_inverse(N)) {
+        if (a.ev) (void)hipEventRecord(a.ev[3], s);  // inverse y is fused into the next pass
+        if (a.t <= 1) launch_inv_yz<N, 1>(a, s);
+        else if (a.t <= 2) launch_inv_yz<N, 2>(a, s);
+        else if (a.t <= 4) launch_inv_yz<N, 4>(a, s);
+        else if (a.t <= 8) launch_inv_yz<N, 8>(a, s);
+        else launch_inv_yz<N, 16>(a, s);
+    } else {
+        kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
+        if (a.ev) (void)hipEventRecord(a.ev[3], s);
+        if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
+        else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
+        else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
+        else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
+        else launch_inv_z<N, L, 16>(a, s);
+    }
     if (a.ev) (void)hipEventRecord(a.ev[4], s);
     return hipGetLastError();
 }
@@ -322,13 +525,8 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
 }  // namespace
 
 int pipe_cands_per_rotation(int nf) {
-    switch (nf) {
-        case 80: return (80 * 80 + 15) / 16;
-        case 96: return (96 * 96 + 15) / 16;
-        case 144: return (144 * 144 + 15) / 16;
-        case 256: return (256 * 256 + 15) / 16;
-        default: return 0;
-    }
+    if (!pipe_available(nf)) return 0;
+    return fused_inverse(nf) ? nf : (nf * nf + 15) / 16;
 }
 
 bool pipe_available(int nf) {

@@ -148,6 +148,35 @@ template <int N> struct Geo {
     static_assert(N1 <= 32, "line too long for the register-resident split");
 };
 
+// General strides: element e of line `line` is S[e * ES + line * LST]; the N-point
+// twiddle w_N^m is tw[m * TWS] (TWS > 1 when tw is a table for a multiple of N).
+template <int N, bool INV>
+__device__ __forceinline__ void step1_g(float2 *v, float2 *S, const float2 *tw, int TWS, int ES, int LST, int line,
+                                        int j2) {
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2;
+    RDft<N1, INV>::run(v);
+    sfor<0, N1>([&](auto kc) {
+        constexpr int k1 = decltype(kc)::value;
+        float2 x = v[k1];
+        if constexpr (k1 != 0) {
+            float2 w = tw[j2 * k1 * TWS];
+            if (INV) w.y = -w.y;
+            x = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        }
+        S[(k1 * N2 + j2) * ES + line * LST] = x;
+    });
+}
+
+template <int N, bool INV>
+__device__ __forceinline__ void step2_g(float2 *y, const float2 *S, int ES, int LST, int line, int k1) {
+    constexpr int N2 = Geo<N>::N2;
+    sfor<0, N2>([&](auto jc) {
+        constexpr int j2 = decltype(jc)::value;
+        y[j2] = S[(k1 * N2 + j2) * ES + line * LST];
+    });
+    RDft<N2, INV>::run(y);
+}
+
 // Step 1 of the four-step transform for one line, from values already in registers:
 // v[j1] = x[j2 + N2 j1]; writes w_N^(j2 k1) * DFT_N1(v)[k1] to S[(k1 N2 + j2) LS + line].
 template <int N, bool INV>
