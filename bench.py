@@ -99,6 +99,10 @@ def main():
     log("context ready: batch %d" % B)
 
     if args.sweep:
+        nbytes = 1 << 30
+        cms = M.copy_probe(nbytes, reps=5)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": round(cms, 4),
+                          "GBps_read_plus_write": round(2 * nbytes / (cms * 1e-3) / 1e9, 1)}), flush=True)
         for item in args.sweep.split(","):
             os.environ["MDK_XCORR_VARIANT"] = item[1:] if item[0] == "v" else os.environ.get("MDK_XCORR_VARIANT", "0")
             bs = int(item[1:]) if item[0] == "b" else B

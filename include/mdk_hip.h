@@ -112,6 +112,11 @@ int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float
  * rotation, i.e. Re IFFT(conj(R) * FFT(ligand)). */
 int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
 
+/* Measurement hook: device-to-device copy of `bytes` (read + write), `reps` times;
+ * *ms = mean milliseconds per copy.  Calibrates HBM counters and the achievable copy
+ * bandwidth.  No reference counterpart. */
+int mdk_copy_probe(int device, size_t bytes, int reps, float *ms);
+
 /* Test hook: `count` independent n-point complex FFTs (interleaved complex64), sign
  * -1 forward / +1 inverse, unnormalised, on device `device`. */
 int mdk_fft1d(int device, int n, int count, const float *in, float *out, int sign);

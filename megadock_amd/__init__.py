@@ -21,7 +21,7 @@ ABI_SYMBOLS = [
     "mdk_set_receptor", "mdk_set_receptor_spectrum", "mdk_get_receptor_spectrum",
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
-    "mdk_score_map", "mdk_fft1d",
+    "mdk_score_map", "mdk_fft1d", "mdk_copy_probe",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -183,6 +183,13 @@ class Context:
         out = np.zeros(self.nf ** 3, np.float32)
         _check(lib().mdk_score_map(self.h, _fp(rc), _fp(out)), "mdk_score_map")
         return out
+
+
+def copy_probe(nbytes, reps=10, device=0):
+    """Milliseconds per device-to-device copy of nbytes (hipMemcpy blit kernel)."""
+    ms = C.c_float(0.0)
+    _check(lib().mdk_copy_probe(C.c_int(device), C.c_size_t(nbytes), C.c_int(reps), C.byref(ms)), "mdk_copy_probe")
+    return ms.value
 
 
 def fft1d(lines, sign=-1, device=0):

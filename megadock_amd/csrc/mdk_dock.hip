@@ -839,6 +839,33 @@ int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
     return rc;
 }
 
+int mdk_copy_probe(int device, size_t bytes, int reps, float *ms) {
+    if (!ms || bytes == 0 || reps < 1) return fail("mdk_copy_probe: invalid argument");
+    HIPCHK(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = fail("mdk_copy_probe: allocation failed");
+    if (!rc && hipMemset(a, 1, bytes) != hipSuccess) rc = fail("mdk_copy_probe: memset failed");
+    if (!rc && hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = fail("mdk_copy_probe: copy failed");
+    if (!rc) {
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++) (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            rc = fail("mdk_copy_probe: timing failed");
+        else
+            *ms /= reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int mdk_fft1d(int device, int n, int count, const float *in, float *out, int sign) {
     if (!in || !out || n < 2 || count < 1) return fail("mdk_fft1d: invalid argument");
     FftPlan p{};
