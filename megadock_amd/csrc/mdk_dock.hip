@@ -238,13 +238,12 @@ __global__ void __launch_bounds__(NT) k_inv_z_top(const float2 *__restrict__ cc,
     }
     // per-thread top-t over its elements; key = value (t == 1) or value / nf^3 (t > 1)
     TopList tl;
-    tl.init(t, -INFINITY);
+    tl.init();
     if (r < nrows) {
         const int rr = r % (nf * nf);  // row within the rotation: x*nf + y
         for (int z = jt; z < nf; z += tpl) {
             const float v = res[z * LS + l].x;
-            const float key = (t == 1) ? v : v / nf3f;
-            tl.push(t, key, rr * nf + z);
+            tl.push(t, (t == 1) ? v : v / nf3f, rr * nf + z);
         }
     }
     Cand *my = lists + ((size_t)l * tpl + jt) * t;
@@ -256,7 +255,7 @@ __global__ void __launch_bounds__(NT) k_inv_z_top(const float2 *__restrict__ cc,
     __syncthreads();
     if (jt == 0 && r < nrows) {
         TopList m;
-        m.init(t, -INFINITY);
+        m.init();
         const Cand *all = lists + (size_t)l * tpl * t;
         for (int q = 0; q < tpl * t; q++) m.push(t, all[q].key, all[q].idx);
         for (int q = 0; q < t; q++) {
@@ -268,56 +267,44 @@ __global__ void __launch_bounds__(NT) k_inv_z_top(const float2 *__restrict__ cc,
 }
 
 // ---------------------------------------------------------------- per-rotation top-t
-// t == 1 starts from (0.0, index 0) and t > 1 from (-99999, index 0), matching the
-// reference's initial values and strict comparisons (src/fft_process.cpp:407-446).
+// Merges a rotation's candidate lists into its top-t, then applies the reference's
+// starting values: t == 1 starts from (0.0, index 0) and keeps a candidate only if its
+// value is > 0; t > 1 starts from -99999 and keeps candidates whose normalised value is
+// > -99999 (src/fft_process.cpp:407-446).  Scores are written normalised by nf^3.
 __global__ void __launch_bounds__(NT) k_reduce_top(const Cand *__restrict__ cand, mdk_sortscore *__restrict__ out,
-                                                   const int *__restrict__ rot_id, int rows_per_rot,
+                                                   const int *__restrict__ rot_id, int lists_per_rot,
                                                    int t, int nf, float nf3f) {
     __shared__ Cand lists[NT * MDK_MAXT];
     const int b = blockIdx.x;
-    const Cand *src = cand + (size_t)b * rows_per_rot * t;
-    const float k0 = (t == 1) ? 0.0f : -99999.0f;
+    const Cand *src = cand + (size_t)b * lists_per_rot * t;
     TopList tl;
-    tl.init(t, k0);
-    const int n = rows_per_rot * t;
+    tl.init();
+    const int n = lists_per_rot * t;
     for (int q = threadIdx.x; q < n; q += NT) tl.push(t, src[q].key, src[q].idx);
-    for (int q = 0; q < t; q++) {
-#pragma unroll
-        for (int s = 0; s < MDK_MAXT; s++)
-            if (s == q) lists[threadIdx.x * t + q] = Cand{tl.k[s], tl.i[s]};
-    }
+    tl.store(t, lists + threadIdx.x * t);
     __syncthreads();
     // tree merge of the NT lists (every thread reaches every barrier)
     for (int half = NT / 2; half >= 1; half >>= 1) {
-        TopList m;
-        m.init(t, k0);
-        if (threadIdx.x < half) {
-            for (int q = 0; q < t; q++) {
-                const Cand a = lists[threadIdx.x * t + q], c = lists[(threadIdx.x + half) * t + q];
-                m.push(t, a.key, a.idx);
-                m.push(t, c.key, c.idx);
-            }
-        }
+        if (threadIdx.x < half) tl.merge_sorted(t, lists + (threadIdx.x + half) * t);
         __syncthreads();
-        if (threadIdx.x < half) {
-            for (int q = 0; q < t; q++) {
-#pragma unroll
-                for (int s = 0; s < MDK_MAXT; s++)
-                    if (s == q) lists[threadIdx.x * t + q] = Cand{m.k[s], m.i[s]};
-            }
-        }
+        if (threadIdx.x < half) tl.store(t, lists + threadIdx.x * t);
         __syncthreads();
     }
-    if (threadIdx.x < t) {
-        const Cand c = lists[threadIdx.x];
-        mdk_sortscore s;
-        s.score = (t == 1) ? c.key / nf3f : c.key;
+    if (threadIdx.x == 0) {
         const int nf2 = nf * nf;
-        s.index[0] = rot_id[b];
-        s.index[1] = c.idx / nf2;
-        s.index[2] = (c.idx / nf) % nf;
-        s.index[3] = c.idx % nf;
-        out[(size_t)b * t + threadIdx.x] = s;
+        bool keep = true;  // the lists are sorted, so the kept entries are a prefix
+        for (int q = 0; q < t; q++) {
+            const Cand c = lists[q];
+            keep = keep && c.idx != kNoIndex && (t == 1 ? c.key > 0.0f : c.key > -99999.0f);
+            const int idx = keep ? c.idx : 0;
+            mdk_sortscore s;
+            s.score = keep ? (t == 1 ? c.key / nf3f : c.key) : (t == 1 ? 0.0f : -99999.0f);
+            s.index[0] = rot_id[b];
+            s.index[1] = idx / nf2;
+            s.index[2] = (idx / nf) % nf;
+            s.index[3] = idx % nf;
+            out[(size_t)b * t + q] = s;
+        }
     }
 }
 

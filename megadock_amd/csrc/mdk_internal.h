@@ -32,48 +32,101 @@ inline size_t vox_scratch_words(int ng) { return 2 * (size_t)mask_words(ng) + co
 
 #define MDK_MAXT 16  // largest t (results per rotation) the device top-t lists hold
 
+// A score candidate: its order key (see better()) and its linear grid index
+// (x*nf + y)*nf + z.
 struct Cand {
     float key;
     int idx;
 };
 
-// candidate order: key descending, then linear index ascending
+// Candidate order of the reference's per-rotation top-t (src/fft_process.cpp:407-446):
+// key descending, then linear index ascending, where the key is the raw value for t == 1
+// (strict '<' from the first index) and fl(value / nf^3) for t > 1 (each value is
+// normalised before the strict '>' insertion).
 __device__ __forceinline__ bool better(float ka, int ia, float kb, int ib) {
     return ka > kb || (ka == kb && ia < ib);
 }
 
-// register top-t list with static indexing (t <= MDK_MAXT); (wk, wi) mirrors the
-// t-th entry so that most candidates are rejected with one comparison
+constexpr int kNoIndex = 0x7fffffff;  // placeholder entries (-inf, kNoIndex) lose to every candidate
+
+// top-TT list with a compile-time length, best first
+template <int TT> struct TopR {
+    float v[TT];
+    int i[TT];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int q = 0; q < TT; q++) {
+            v[q] = -INFINITY;
+            i[q] = kNoIndex;
+        }
+    }
+    __device__ __forceinline__ void push(float vc, int ic) {
+        if (!better(vc, ic, v[TT - 1], i[TT - 1])) return;
+#pragma unroll
+        for (int q = 0; q < TT; q++) {
+            if (better(vc, ic, v[q], i[q])) {
+                const float tv = v[q];
+                const int ti = i[q];
+                v[q] = vc;
+                i[q] = ic;
+                vc = tv;
+                ic = ti;
+            }
+        }
+    }
+    __device__ __forceinline__ void pop() {
+#pragma unroll
+        for (int q = 0; q + 1 < TT; q++) {
+            v[q] = v[q + 1];
+            i[q] = i[q + 1];
+        }
+        v[TT - 1] = -INFINITY;
+        i[TT - 1] = kNoIndex;
+    }
+};
+
+// Top-t of the sorted lists held by the 64 lanes of a wavefront, by t rounds of a
+// butterfly arg-max over the list heads; every lane receives the result in (ov, oi).
+template <int TT>
+__device__ __forceinline__ void wave_top(TopR<TT> &L, int t, float *ov, int *oi) {
+#pragma unroll
+    for (int r = 0; r < TT; r++) {
+        if (r >= t) break;  // uniform
+        float bv = L.v[0];
+        int bi = L.i[0];
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const float pv = __shfl_xor(bv, m);
+            const int pi = __shfl_xor(bi, m);
+            if (better(pv, pi, bv, bi)) {
+                bv = pv;
+                bi = pi;
+            }
+        }
+        ov[r] = bv;
+        oi[r] = bi;
+        if (L.v[0] == bv && L.i[0] == bi) L.pop();
+    }
+}
+
+// register top-t list with a run-time t <= MDK_MAXT (static indexing); (wv, wi) mirrors
+// the t-th entry so that most candidates are rejected with one comparison
 struct TopList {
     float k[MDK_MAXT];
     int i[MDK_MAXT];
-    float wk;
+    float wv;
     int wi;
-    __device__ void init(int t, float k0) {
+    __device__ void init() {
 #pragma unroll
         for (int q = 0; q < MDK_MAXT; q++) {
-            k[q] = k0;
-            i[q] = 0;
+            k[q] = -INFINITY;
+            i[q] = kNoIndex;
         }
-        wk = k0;
-        wi = 0;
-    }
-    // insert another list sorted best-first; stops at its first rejected entry
-    __device__ void merge_sorted(int t, const Cand *other) {
-        for (int q = 0; q < t; q++) {
-            if (!better(other[q].key, other[q].idx, wk, wi)) break;
-            push(t, other[q].key, other[q].idx);
-        }
-    }
-    __device__ void store(int t, Cand *dst) const {
-        for (int q = 0; q < t; q++) {
-#pragma unroll
-            for (int s = 0; s < MDK_MAXT; s++)
-                if (s == q) dst[q] = Cand{k[s], i[s]};
-        }
+        wv = -INFINITY;
+        wi = kNoIndex;
     }
     __device__ void push(int t, float kc, int ic) {
-        if (!better(kc, ic, wk, wi)) return;
+        if (!better(kc, ic, wv, wi)) return;
 #pragma unroll
         for (int q = 0; q < MDK_MAXT; q++) {
             if (q < t && better(kc, ic, k[q], i[q])) {
@@ -85,52 +138,24 @@ struct TopList {
                 ic = ti;
             }
             if (q == t - 1) {
-                wk = k[q];
+                wv = k[q];
                 wi = i[q];
             }
         }
     }
-};
-
-
-// top-TT list with a compile-time length: key descending, index ascending
-template <int TT> struct TopT {
-    float k[TT];
-    int i[TT];
-    __device__ __forceinline__ void init(float k0) {
-#pragma unroll
-        for (int q = 0; q < TT; q++) {
-            k[q] = k0;
-            i[q] = 0;
+    // insert another list sorted best-first; stops at its first rejected entry
+    __device__ void merge_sorted(int t, const Cand *other) {
+        for (int q = 0; q < t; q++) {
+            if (!better(other[q].key, other[q].idx, wv, wi)) break;
+            push(t, other[q].key, other[q].idx);
         }
     }
-    __device__ __forceinline__ void push(float kc, int ic) {
-        if (!better(kc, ic, k[TT - 1], i[TT - 1])) return;
+    __device__ void store(int t, Cand *dst) const {
+        for (int q = 0; q < t; q++) {
 #pragma unroll
-        for (int q = 0; q < TT; q++) {
-            if (better(kc, ic, k[q], i[q])) {
-                const float tk = k[q];
-                const int ti = i[q];
-                k[q] = kc;
-                i[q] = ic;
-                kc = tk;
-                ic = ti;
-            }
+            for (int s = 0; s < MDK_MAXT; s++)
+                if (s == q) dst[q] = Cand{k[s], i[s]};
         }
-    }
-    // merge with the list of lane (lane ^ mask) of the same wavefront; partner_ok must be
-    // false when that lane does not exist (a partial last wavefront of the block)
-    __device__ __forceinline__ void merge_xor(int mask, bool partner_ok) {
-        float ok[TT];
-        int oi[TT];
-#pragma unroll
-        for (int q = 0; q < TT; q++) {
-            ok[q] = __shfl_xor(k[q], mask);
-            oi[q] = __shfl_xor(i[q], mask);
-        }
-        if (partner_ok)
-#pragma unroll
-            for (int q = 0; q < TT; q++) push(ok[q], oi[q]);
     }
 };
 

@@ -223,6 +223,33 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_inv_y(float2 *__restrict__ cc
     }
 }
 
+// Block-wide top-t: each wavefront extracts the top-t of its lanes' lists, lane 0 of each
+// wavefront leaves them in wl, and thread 0 merges the NW short lists and writes t
+// candidates to dst.
+template <int TT, int NW>
+__device__ __forceinline__ void block_top(TopR<TT> &tl, int t, Cand *wl, Cand *dst) {
+    float ov[TT];
+    int oi[TT];
+    wave_top<TT>(tl, t, ov, oi);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < TT; q++)
+            if (q < t) wl[(tid >> 6) * TT + q] = Cand{ov[q], oi[q]};
+    __syncthreads();
+    if (tid == 0) {
+        TopR<TT> m;
+        m.init();
+        for (int w = 0; w < NW; w++)
+#pragma unroll
+            for (int q = 0; q < TT; q++)
+                if (q < t) m.push(wl[w * TT + q].key, wl[w * TT + q].idx);
+#pragma unroll
+        for (int q = 0; q < TT; q++)
+            if (q < t) dst[q] = Cand{m.v[q], m.i[q]};
+    }
+}
+
 // ------------------------------------------------------------------ inverse z + top-t
 // block (row chunk, b): rows y0..y0+L-1 of rotation b, row r = x*N + y; the row's
 // spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps the top-TT
@@ -235,6 +262,7 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
                  const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
     constexpr int NROW = N * N, NW = (NT + 63) / 64;
+    static_assert(NT % 64 == 0, "the wavefront top-t needs whole wavefronts");
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
@@ -262,8 +290,10 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
     }
     __syncthreads();
     const int r = row0 + line;
-    TopT<TT> tl;
-    tl.init(-INFINITY);
+    // keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1)
+    constexpr bool QUOT = TT > 1;
+    TopR<TT> tl;
+    tl.init();
     if (u < N1) {
         float2 y[N2];
         rf::step2_load<N, true>(y, S, LS, line, u);
@@ -274,31 +304,15 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
                     smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
                 });
             } else {
-                const bool raw = (t == 1);
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
-                    const float v = y[k2].x;
-                    tl.push(raw ? v : v / nf3f, r * N + u + N1 * k2);
+                    tl.push(QUOT ? y[k2].x / nf3f : y[k2].x, r * N + u + N1 * k2);
                 });
             }
         }
     }
     if (smap) return;  // uniform
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) tl.merge_xor(m, (tid ^ m) < NT);
-    if ((tid & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < TT; q++) wl[(tid >> 6) * TT + q] = Cand{tl.k[q], tl.i[q]};
-    __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < NW; w++)
-#pragma unroll
-            for (int q = 0; q < TT; q++) tl.push(wl[w * TT + q].key, wl[w * TT + q].idx);
-        Cand *dst = cand + ((size_t)b * gridDim.x + blockIdx.x) * t;
-#pragma unroll
-        for (int q = 0; q < TT; q++)
-            if (q < t) dst[q] = Cand{tl.k[q], tl.i[q]};
-    }
+    block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
 template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
@@ -404,8 +418,10 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
-    TopT<TT> tl;
-    tl.init(-INFINITY);
+    // keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1)
+    constexpr bool QUOT = TT > 1;
+    TopR<TT> tl;
+    tl.init();
     {
         float2 v[KZ][N1z];
 #pragma unroll
@@ -424,7 +440,6 @@ __global__ void __launch_bounds__(1024)
             if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
         }
         __syncthreads();
-        const bool raw = (t == 1);
 #pragma unroll
         for (int j = 0; j < KZ; j++) {
             const int task = tid + j * NTH, line = task % N, u = task / N;
@@ -443,29 +458,15 @@ __global__ void __launch_bounds__(1024)
                     sfor<0, N2z>([&](auto kc) {
                         constexpr int k2 = decltype(kc)::value;
                         const int m = u + N1z * k2;
-                        tl.push(raw ? y[k2].x : y[k2].x / nf3f, base + 2 * m);
-                        tl.push(raw ? y[k2].y : y[k2].y / nf3f, base + 2 * m + 1);
+                        tl.push(QUOT ? y[k2].x / nf3f : y[k2].x, base + 2 * m);
+                        tl.push(QUOT ? y[k2].y / nf3f : y[k2].y, base + 2 * m + 1);
                     });
                 }
             }
         }
     }
     if (smap) return;  // uniform
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) tl.merge_xor(m, true);
-    if ((tid & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < TT; q++) wl[(tid >> 6) * TT + q] = Cand{tl.k[q], tl.i[q]};
-    __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < NW; w++)
-#pragma unroll
-            for (int q = 0; q < TT; q++) tl.push(wl[w * TT + q].key, wl[w * TT + q].idx);
-        Cand *dst = cand + ((size_t)b * N + x) * t;
-#pragma unroll
-        for (int q = 0; q < TT; q++)
-            if (q < t) dst[q] = Cand{tl.k[q], tl.i[q]};
-    }
+    block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * N + x) * t);
 }
 
 template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
