@@ -30,7 +30,7 @@ int auto_batch(int nf, int ng, int t) {
     const double H = nf / 2 + 1;
     const double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
                                   (double)nf * nf * t);
-    int b = (int)(2.0 * 1024 * 1024 * 1024 / per_rot);
+    int b = (int)(8.0 * 1024 * 1024 * 1024 / per_rot);  // 8 GiB of batch buffers
     return std::max(1, std::min(128, b));
 }
 

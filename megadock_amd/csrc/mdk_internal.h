@@ -87,11 +87,10 @@ template <int TT> struct TopR {
 
 // Top-t of the sorted lists held by the 64 lanes of a wavefront, by t rounds of a
 // butterfly arg-max over the list heads; every lane receives the result in (ov, oi).
-template <int TT>
+template <int TT, int R = 0>
 __device__ __forceinline__ void wave_top(TopR<TT> &L, int t, float *ov, int *oi) {
-#pragma unroll
-    for (int r = 0; r < TT; r++) {
-        if (r >= t) break;  // uniform
+    if constexpr (R < TT) {  // one round per rank, unrolled at compile time
+        if (R >= t) return;  // uniform
         float bv = L.v[0];
         int bi = L.i[0];
 #pragma unroll
@@ -103,10 +102,43 @@ __device__ __forceinline__ void wave_top(TopR<TT> &L, int t, float *ov, int *oi)
                 bi = pi;
             }
         }
-        ov[r] = bv;
-        oi[r] = bi;
+        ov[R] = bv;
+        oi[R] = bi;
         if (L.v[0] == bv && L.i[0] == bi) L.pop();
+        wave_top<TT, R + 1>(L, t, ov, oi);
     }
+}
+
+// The t-th largest of the lane values m across the wavefront (counting repeats; -inf if
+// fewer than t lanes hold a value).  With m = each lane's largest score, at least t of the
+// wavefront's scores are >= the result.  Every lane of the wavefront must call it.
+__device__ __forceinline__ float wave_kth_max(float m, int t) {
+    float tau = -INFINITY;
+    const int lane = (int)(threadIdx.x & 63);
+    for (int r = 0; r < t; r++) {  // t is uniform
+        float w = m;
+        w = fmaxf(w, __shfl_xor(w, 32));
+        w = fmaxf(w, __shfl_xor(w, 16));
+        w = fmaxf(w, __shfl_xor(w, 8));
+        w = fmaxf(w, __shfl_xor(w, 4));
+        w = fmaxf(w, __shfl_xor(w, 2));
+        w = fmaxf(w, __shfl_xor(w, 1));
+        tau = w;
+        const unsigned long long holders = __ballot(m == w);
+        if (holders && lane == __ffsll((long long)holders) - 1) m = -INFINITY;  // retire one holder
+    }
+    return tau;
+}
+
+// Largest float f such that every v < f has fl(v / c) < fl(tau / c), for c > 0 with c
+// exact in 24 bits: f <= c * prev(fl(tau / c)) and the product is exact in double.
+// Scores below f cannot tie with tau after the division, so they lose to it.
+__device__ __forceinline__ float quotient_cut(float tau, float c) {
+    const float qp = nextafterf(tau / c, -INFINITY);
+    const double d = (double)c * (double)qp;
+    float f = (float)d;
+    if ((double)f > d) f = nextafterf(f, -INFINITY);
+    return f;
 }
 
 // register top-t list with a run-time t <= MDK_MAXT (static indexing); (wv, wi) mirrors

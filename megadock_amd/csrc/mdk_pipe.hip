@@ -290,28 +290,34 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
     }
     __syncthreads();
     const int r = row0 + line;
-    // keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1)
+    const bool act = u < N1 && r < NROW;
+    float2 y[N2];
+    if (u < N1) rf::step2_load<N, true>(y, S, LS, line, u);
+    if (smap) {  // uniform
+        if (act)
+            sfor<0, N2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
+            });
+        return;
+    }
+    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
+    // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
+    // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
     constexpr bool QUOT = TT > 1;
+    float mx = -INFINITY;
+    if (act)
+        sfor<0, N2>([&](auto kc) { mx = fmaxf(mx, y[decltype(kc)::value].x); });
+    const float tau = wave_kth_max(mx, t);
+    const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
     TopR<TT> tl;
     tl.init();
-    if (u < N1) {
-        float2 y[N2];
-        rf::step2_load<N, true>(y, S, LS, line, u);
-        if (r < NROW) {
-            if (smap) {
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
-                });
-            } else {
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    tl.push(QUOT ? y[k2].x / nf3f : y[k2].x, r * N + u + N1 * k2);
-                });
-            }
-        }
-    }
-    if (smap) return;  // uniform
+    if (act)
+        sfor<0, N2>([&](auto kc) {
+            constexpr int k2 = decltype(kc)::value;
+            const float v = y[k2].x;
+            if (v >= cut) tl.push(QUOT ? v / nf3f : v, r * N + u + N1 * k2);
+        });
     block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
@@ -418,10 +424,7 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
-    // keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1)
-    constexpr bool QUOT = TT > 1;
-    TopR<TT> tl;
-    tl.init();
+    float2 y[KZ][N2z];
     {
         float2 v[KZ][N1z];
 #pragma unroll
@@ -443,29 +446,54 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
         for (int j = 0; j < KZ; j++) {
             const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N1z) {
-                float2 y[N2z];
-                rf::step2_g<M, true>(y, P, 1, LSP, line, u);
-                const int base = (x * N + line) * N;
-                if (smap) {
-                    sfor<0, N2z>([&](auto kc) {
-                        constexpr int k2 = decltype(kc)::value;
-                        const int m = u + N1z * k2;
-                        smap[(size_t)b * N * N * N + base + 2 * m] = y[k2].x;
-                        smap[(size_t)b * N * N * N + base + 2 * m + 1] = y[k2].y;
-                    });
-                } else {
-                    sfor<0, N2z>([&](auto kc) {
-                        constexpr int k2 = decltype(kc)::value;
-                        const int m = u + N1z * k2;
-                        tl.push(QUOT ? y[k2].x / nf3f : y[k2].x, base + 2 * m);
-                        tl.push(QUOT ? y[k2].y / nf3f : y[k2].y, base + 2 * m + 1);
-                    });
-                }
-            }
+            if (u < N1z) rf::step2_g<M, true>(y[j], P, 1, LSP, line, u);
         }
     }
-    if (smap) return;  // uniform
+    // z = 2m and 2m+1 of row `line` are y[j][k2].x and .y with m = u + N1z k2
+    if (smap) {  // uniform
+#pragma unroll
+        for (int j = 0; j < KZ; j++) {
+            const int task = tid + j * NTH, line = task % N, u = task / N;
+            if (u < N1z)
+                sfor<0, N2z>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const size_t o = (size_t)b * N * N * N + (size_t)(x * N + line) * N + 2 * (u + N1z * k2);
+                    smap[o] = y[j][k2].x;
+                    smap[o + 1] = y[j][k2].y;
+                });
+        }
+        return;
+    }
+    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
+    // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
+    // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
+    constexpr bool QUOT = TT > 1;
+    float mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < KZ; j++) {
+        if ((tid + j * NTH) / N < N1z)
+            sfor<0, N2z>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
+            });
+    }
+    const float tau = wave_kth_max(mx, t);
+    const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
+    TopR<TT> tl;
+    tl.init();
+#pragma unroll
+    for (int j = 0; j < KZ; j++) {
+        const int task = tid + j * NTH, line = task % N, u = task / N;
+        if (u < N1z) {
+            const int base = (x * N + line) * N + 2 * u;
+            sfor<0, N2z>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                const float v0 = y[j][k2].x, v1 = y[j][k2].y;
+                if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, base + 2 * N1z * k2);
+                if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, base + 2 * N1z * k2 + 1);
+            });
+        }
+    }
     block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * N + x) * t);
 }
 
@@ -483,9 +511,15 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[0], s);
     kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4, ng, nd);
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
-    // correlation-pass variant: receptor tile cached in registers for 4 rotations (0),
-    // re-read from L2 for each of 4 rotations (1), one rotation per workgroup (2)
+    // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
+    // the grid, so consecutive workgroups share the receptor tile in L2 (3, the default);
+    // receptor tile cached in registers for 4 rotations (0); re-read from L2 for each of 4
+    // rotations (1); one rotation per workgroup, tile index fastest (2)
     switch (a.variant) {
+        case 0:
+            kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
+            break;
         case 1:
             kp_x_corr<N, LX, 4, false><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
                 a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
@@ -494,13 +528,9 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
             kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng, nd,
                                                                                    a.H, a.Hp, B);
             break;
-        case 3:
+        default:
             kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng,
                                                                                         nd, a.H, a.Hp, B);
-            break;
-        default:
-            kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
     }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     if constexpr (fused_inverse(N)) {

@@ -217,3 +217,26 @@ def test_dock_range_matches_batches_and_topn(gpu, gcq, kernels):
     order = sorted(range(len(res)), key=lambda i: (-res[i]["score"], i))[:50]
     assert np.array_equal(top.view(np.uint8), res[order].view(np.uint8))
     ctx.close()
+
+
+@pytest.mark.parametrize("t", [1, 3])
+@pytest.mark.parametrize("ng", [40, 72, 128])
+def test_dock_all_tied_scores(gpu, gcq, kernels, t, ng):
+    """A zero receptor spectrum makes every score exactly 0: the reference's rules give
+    (0, index 0) for t == 1 (nothing beats the starting 0.0) and the first t grid indices
+    for t > 1 (0 > -99999, ties kept in index order), src/fft_process.cpp:407-455."""
+    rec, lig, _ = gcq
+    nf = 2 * ng
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=4)
+    z = np.zeros(nf ** 3, np.float32)
+    ctx.set_receptor_spectrum(z, z)
+    ang = O.load_rotations(24)
+    ids = np.array([3, 17, 5], np.int32)
+    _say("all-tied dock t=%d nf=%d (%s)" % (t, nf, kernels))
+    got = ctx.dock(M.rotation_coefs(ang[ids]), ids)
+    for q in range(len(ids)):
+        for k in range(t):
+            g = got[q * t + k]
+            assert g["score"] == 0.0, (q, k, g)
+            assert list(g["index"]) == [int(ids[q]), 0, 0, k], (q, k, g)
+    ctx.close()
