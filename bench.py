@@ -63,8 +63,9 @@ def main():
     ap.add_argument("--cpu-rotations", type=int, default=0, help="CPU baseline sample size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
-    ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants and batch "
-                    "sizes, e.g. 'v0,v1,v3,b32,b128'; prints one stage-timing line per setting and exits")
+    ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i) and "
+                    "batch sizes (b), e.g. 'v0,v3,i1,b64'; settings accumulate; prints one stage-timing line "
+                    "per setting and exits")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -104,7 +105,10 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes, "ms": round(cms, 4),
                           "GBps_read_plus_write": round(2 * nbytes / (cms * 1e-3) / 1e9, 1)}), flush=True)
         for item in args.sweep.split(","):
-            os.environ["MDK_XCORR_VARIANT"] = item[1:] if item[0] == "v" else os.environ.get("MDK_XCORR_VARIANT", "0")
+            if item[0] == "v":
+                os.environ["MDK_XCORR_VARIANT"] = item[1:]
+            if item[0] == "i":
+                os.environ["MDK_INV_VARIANT"] = item[1:]
             bs = int(item[1:]) if item[0] == "b" else B
             sctx = M.Context(nf, ng, lig.na, t=t, max_batch=bs)
             sctx.set_receptor(gr, gi)
@@ -161,6 +165,17 @@ def main():
 
     b_rot = 32.0 * nf ** 3
     achieved = rot_s * b_rot / 1e9
+    # measured HBM bytes per rotation of the same pipeline (rocprofv3 FETCH_SIZE/WRITE_SIZE,
+    # copy-calibrated; written by `python -m megadock_amd.profiling traffic`)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_%s_nf%d.json" % (args.workload, nf))
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tr = json.load(fh)
+        traffic = {"bytes_per_batch": round(tr["bytes_per_rotation"] * min(B, R)),
+                   "bytes_per_rotation": round(tr["bytes_per_rotation"]),
+                   "GBps_at_this_rate": round(tr["bytes_per_rotation"] * rot_s / 1e9, 1),
+                   "source": os.path.relpath(tfile, ROOT)}
     out = {
         "metric": METRIC,
         "value": round(rot_s, 1),
@@ -179,7 +194,9 @@ def main():
                                % (args.workload, cfg["rec_res"], rec.na, cfg["lig_res"], lig.na, nf, R, t, cfg["n_out"]),
                    "nf": nf, "ng": ng, "rotations": R, "t": t, "batch": B},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_batch"] if traffic else None,
+                     "traffic_detail": traffic,
                      "unit_of_work": "one batch of %d rotations (the per-batch pipeline of %d kernels on one stream)"
                                      % (nb, 10),
                      "bytes_per_rotation": b_rot, "dominant_kernel": dominant,

@@ -387,6 +387,8 @@ struct mdk_ctx {
     float2 *d_tw4 = nullptr;  // exp(-2 pi i m / nf): twiddles of the specialised pipeline
     bool use_pipe = false;    // compile-time specialised kernels exist for nf
     int xcorr_variant = 3;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
+    int cu_count = 256;       // compute units (persistent grids)
+    int inv_variant = 0;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
     float2 *d_rec = nullptr;
     bool have_rec = false, have_lig = false;
     // ligand
@@ -399,7 +401,9 @@ struct mdk_ctx {
     float *d_brot = nullptr; // staging for mdk_dock (max_batch * 9)
     int *d_bid = nullptr;    // staging ids
     // batch buffers
-    uint32_t *d_bits = nullptr;
+    uint32_t *d_bits = nullptr;  // voxelization scratch, laid out by vs
+    VoxScratch vs{};
+    float *d_chg = nullptr;      // charge grids, max_batch * ng^3, zero between batches
     int *d_vidx = nullptr;
     float2 *d_lg = nullptr, *d_a1 = nullptr, *d_a2 = nullptr, *d_cc = nullptr;
     Cand *d_cand = nullptr;
@@ -414,8 +418,13 @@ size_t lds_bytes(const mdk_ctx *c, int lines) {
     return sizeof(float2) * ((size_t)c->plan.ntw + 2 * (size_t)c->nf * (lines + 1));
 }
 
-// Run the forward ligand passes + correlation + inverse passes for B rotations whose
-// voxel grids are already in d_lg; output top-t candidates (or the score map).
+VoxGrid vox_grid(const mdk_ctx *c) {
+    const int ng = c->ng;
+    return VoxGrid{c->vs.dmo, c->d_chg, ng, row_words(ng), mask_words(ng), ng * ng * ng, c->lig.delta};
+}
+
+// Run the forward ligand passes + correlation + inverse passes for B rotations that
+// launch_voxelize has voxelized; output top-t candidates (or the score map).
 // ev (optional): events recorded after each of the five FFT-side stages
 int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
@@ -429,7 +438,7 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
         a.Hp = c->Hp;
         a.t = c->t;
         a.nf3f = (float)((size_t)nf * nf * nf);
-        a.lg = c->d_lg;
+        a.vox = vox_grid(c);
         a.a1 = c->d_a1;
         a.a2 = c->d_a2;
         a.cc = c->d_cc;
@@ -440,9 +449,12 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
         a.stream = s;
         a.ev = ev;
         a.variant = c->xcorr_variant;
+        a.cu_count = c->cu_count;
+        a.inv_variant = c->inv_variant;
         HIPCHK(pipe_run(nf, a));
         return 0;
     }
+    HIPCHK(launch_vox_expand(vox_grid(c), B, c->d_lg, s));
     const int lines_z = B * ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
@@ -465,7 +477,7 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
 int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev,
                hipEvent_t *ev = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-    HIPCHK(launch_voxelize(c->lig, rot_dev, B, c->d_bits, c->d_vidx, c->d_lg, c->stream));
+    HIPCHK(launch_voxelize(c->lig, rot_dev, B, c->vs, c->d_vidx, c->d_chg, c->stream));
     if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
     if (run_fft_pipeline(c, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
@@ -551,10 +563,16 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
         dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) ||
         dmalloc(&c->d_bits, B * vox_scratch_words(ng)) || dmalloc(&c->d_vidx, B * na) ||
-        dmalloc(&c->d_lg, B * ng3) || dmalloc(&c->d_a1, B * ng * ng * nf) ||
+        dmalloc(&c->d_lg, B * ng3) || dmalloc(&c->d_chg, B * ng3) || dmalloc(&c->d_a1, B * ng * ng * nf) ||
         dmalloc(&c->d_a2, B * ng * nf * nf) || dmalloc(&c->d_cc, B * nf * nf * c->Hp) ||
         dmalloc(&c->d_cand, B * nf * nf * t) || dmalloc(&c->d_bout, B * t))
         return bail(1);
+    c->vs = vox_scratch(c->d_bits, B, ng);
+    // the atom counts and the charge grids start, and are kept between batches, all-zero
+    if (hipMemsetAsync(c->d_bits, 0, sizeof(uint32_t) * B * vox_scratch_words(ng), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_chg, 0, sizeof(float) * B * ng3, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail("mdk_create: scratch initialisation failed"));
     if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail("mdk_create: twiddle upload failed"));
     {
@@ -570,6 +588,11 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
         c->xcorr_variant = xv ? std::atoi(xv) : 3;
+        const char *iv = std::getenv("MDK_INV_VARIANT");
+        c->inv_variant = iv ? std::atoi(iv) : 0;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+            c->cu_count = cus;
     }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail("mdk_create: event creation failed"));
@@ -590,7 +613,7 @@ int mdk_destroy(mdk_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
-                    c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
+                    c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_chg, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
                     c->d_cc, c->d_cand, c->d_bout, c->d_results};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -795,7 +818,8 @@ int mdk_voxelize(mdk_ctx *c, const float *rotcoef, int B, float *grid_r, float *
     if (B < 1 || B > c->max_batch) return fail("mdk_voxelize: batch size out of range");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, c->d_bits, c->d_vidx, c->d_lg, c->stream));
+    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, c->vs, c->d_vidx, c->d_chg, c->stream));
+    HIPCHK(launch_vox_expand(vox_grid(c), B, c->d_lg, c->stream));
     const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
     std::vector<float2> h(B * ng3);
     HIPCHK(hipMemcpyAsync(h.data(), c->d_lg, sizeof(float2) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -816,7 +840,7 @@ int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
     HIPCHK(hipMalloc(&d_map, sizeof(float) * nf3));
     int rc = 0;
     if (hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        launch_voxelize(c->lig, c->d_brot, 1, c->d_bits, c->d_vidx, c->d_lg, c->stream) != hipSuccess)
+        launch_voxelize(c->lig, c->d_brot, 1, c->vs, c->d_vidx, c->d_chg, c->stream) != hipSuccess)
         rc = fail("mdk_score_map: voxelization failed");
     if (!rc) rc = run_fft_pipeline(c, 1, d_map);
     if (!rc && (hipMemcpyAsync(out, d_map, sizeof(float) * nf3, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

@@ -18,17 +18,52 @@ struct LigandDev {
     const float *gc;     // ng grid coordinates
 };
 
-// Ligand voxelization of B rotations into lg[b*ng^3 + v] = (grid_r, grid_i).
-// rot: B*9 rotation coefficients (row-major r11..r33).  bits: B * vox_scratch_words(ng)
-// words (core and surface bit masks, per-voxel atom counts), vidx: B*na ints; scratch.
-hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, uint32_t *bits,
-                           int *vidx, float2 *lg, hipStream_t s);
-
-inline int mask_words(int ng) { return (ng * ng * ng + 31) / 32; }
+// Voxel masks are stored row by row: row (i, j) of ng voxels along k in row_words(ng)
+// 32-bit words, bit k & 31 of word k >> 5.
+__host__ __device__ inline int row_words(int ng) { return (ng + 31) / 32; }
+inline int mask_words(int ng) { return ng * ng * row_words(ng); }
 // per-voxel atom counts, two 16-bit counters per word (ligands have < 65536 atoms)
 inline int count_words(int ng) { return (ng * ng * ng + 1) / 2; }
-// scratch words launch_voxelize needs per rotation (core bits, surface bits, counts)
-inline size_t vox_scratch_words(int ng) { return 2 * (size_t)mask_words(ng) + count_words(ng); }
+// scratch words per rotation: core + surface masks, delta + value-1 masks, counts
+inline size_t vox_scratch_words(int ng) { return 4 * (size_t)mask_words(ng) + count_words(ng); }
+
+// Voxelization scratch of a context, laid out for max_batch rotations:
+//   cs  [b][2][mask_words]  core and surface bits (cleared per batch)
+//   dmo [b][2][mask_words]  delta mask D and value-1 mask O (written per batch)
+//   cnt [b][count_words]    atom counts per voxel, zero between batches
+struct VoxScratch {
+    uint32_t *cs, *dmo, *cnt;
+};
+inline VoxScratch vox_scratch(uint32_t *base, int max_batch, int ng) {
+    const size_t mw = mask_words(ng);
+    return VoxScratch{base, base + 2 * mw * max_batch, base + 4 * mw * max_batch};
+}
+
+// A batch of ligand grids as the FFT passes read them: shape value delta where D is set,
+// 1 where O is set, 0 elsewhere; charge from the dense grid chg[b][ng^3], which is zero
+// between batches (vox_take clears each voxel it reads).
+struct VoxGrid {
+    const uint32_t *dmo;
+    float *chg;
+    int ng, rw, mw, ng3;
+    float delta;
+};
+__device__ __forceinline__ float2 vox_take(const VoxGrid &g, int b, int row, int k) {
+    const uint32_t *d = g.dmo + (size_t)b * 2 * g.mw + row * g.rw + (k >> 5);
+    const uint32_t bit = 1u << (k & 31);
+    const float sv = (d[0] & bit) ? g.delta : ((d[g.mw] & bit) ? 1.0f : 0.0f);
+    float *cp = g.chg + (size_t)b * g.ng3 + (size_t)row * g.ng + k;
+    const float q = *cp;
+    if (__float_as_uint(q) != 0u) *cp = 0.0f;
+    return make_float2(sv, q);
+}
+
+// Ligand voxelization of B rotations (rot: B*9 rotation coefficients, row-major r11..r33)
+// into the masks of vs and the charge grid chg (B * ng^3, zero on entry); vidx: B*na ints.
+hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const VoxScratch &vs, int *vidx,
+                           float *chg, hipStream_t s);
+// (grid_r, grid_i) pairs lg[b*ng^3 + v] from a voxelized batch; clears its charge grid
+hipError_t launch_vox_expand(const VoxGrid &vg, int B, float2 *lg, hipStream_t s);
 
 #define MDK_MAXT 16  // largest t (results per rotation) the device top-t lists hold
 
@@ -195,7 +230,7 @@ struct TopList {
 struct PipeArgs {
     int B, ng, nd, H, Hp, t;
     float nf3f;
-    const float2 *lg;
+    VoxGrid vox;        // voxelized ligands of the batch
     float2 *a1, *a2, *cc;
     const float2 *rec;
     Cand *cand;
@@ -204,6 +239,8 @@ struct PipeArgs {
     hipStream_t stream;
     hipEvent_t *ev;     // optional: 5 events, one after each pass
     int variant;        // kernel variant of the correlation pass (MDK_XCORR_VARIANT)
+    int cu_count;       // compute units of the device (persistent grids)
+    int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches (MDK_INV_VARIANT)
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.

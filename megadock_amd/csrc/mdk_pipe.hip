@@ -14,13 +14,25 @@ namespace {
 using rf::Geo;
 using rf::sfor;
 
+// Global accesses as a block-uniform base plus a 32-bit byte offset per lane, so that
+// the address is formed by the load/store itself (scalar base + vector offset) instead
+// of 64-bit vector arithmetic per access.
+template <class T> __device__ __forceinline__ T ld_at(const T *base, unsigned byte_off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <class T> __device__ __forceinline__ void st_at(T *base, unsigned byte_off, const T &v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
 
 // ------------------------------------------------------------------ forward z
-// lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1
+// lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1;
+// their values come straight from the voxel masks and the charge grid
 template <int N, int L>
-__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(const float2 *__restrict__ lg, float2 *__restrict__ a1,
-                                                          const float2 *__restrict__ twg, int ng, int nd, int nlines) {
+__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(VoxGrid vg, float2 *__restrict__ a1,
+                                                          const float2 *__restrict__ twg, int nlines) {
     constexpr int T = Geo<N>::T, LS = L + 1, NT = L * T;
+    constexpr int NG = N / 2, ND = N / 4;  // ligand voxels per axis and their offset in the line
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
@@ -29,14 +41,18 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(const float2 *__restric
     for (int q = tid; q < L * N; q += NT) {
         const int l = q / N, e = q % N;
         float2 v = make_float2(0.f, 0.f);
-        if (line0 + l < nlines && e >= nd && e < nd + ng) v = lg[(size_t)(line0 + l) * ng + (e - nd)];
+        if (line0 + l < nlines && e >= ND && e < ND + NG) {
+            const int b = (line0 + l) / (NG * NG), row = line0 + l - b * NG * NG;
+            v = vox_take(vg, b, row, e - ND);
+        }
         S[e * LS + l] = v;
     }
     __syncthreads();
     rf::four_step_inplace<N, L, false>(S, tw, LS, line, u, true);
+    float2 *dst = a1 + (size_t)line0 * N;  // uniform base, 32-bit lane offsets
     for (int q = tid; q < L * N; q += NT) {
         const int l = q / N, e = q % N;
-        if (line0 + l < nlines) a1[(size_t)(line0 + l) * N + e] = S[e * LS + l];
+        if (line0 + l < nlines) st_at(dst, 8u * (unsigned)q, S[e * LS + l]);
     }
 }
 
@@ -44,22 +60,23 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(const float2 *__restric
 // block (kz chunk, x, b); rows y < ng of a1 sit at y + nd; loads straight to registers
 template <int N, int L>
 __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restrict__ a1, float2 *__restrict__ a2,
-                                                          const float2 *__restrict__ twg, int ng, int nd) {
+                                                          const float2 *__restrict__ twg) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    constexpr int NG = N / 2, ND = N / 4;
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
     const int kz = blockIdx.x * L + line, x = blockIdx.y, b = blockIdx.z;
     const bool col = kz < N;
-    const float2 *src = a1 + ((size_t)b * ng + x) * ng * N;
+    const float2 *src = a1 + ((size_t)b * NG + x) * NG * N;
     __syncthreads();
     if (u < N2) {
         float2 v[N1];
         sfor<0, N1>([&](auto jc) {
             constexpr int j1 = decltype(jc)::value;
-            const int y = u + N2 * j1 - nd;
-            v[j1] = (col && y >= 0 && y < ng) ? src[(size_t)y * N + kz] : make_float2(0.f, 0.f);
+            const int y = u + N2 * j1 - ND;
+            v[j1] = (col && y >= 0 && y < NG) ? ld_at(src, 8u * (unsigned)(y * N + kz)) : make_float2(0.f, 0.f);
         });
         rf::step1_store<N, false>(v, S, tw, LS, line, u);
     }
@@ -67,11 +84,11 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
     if (u < N1) {
         float2 y[N2];
         rf::step2_load<N, false>(y, S, LS, line, u);
-        float2 *dst = a2 + ((size_t)b * ng + x) * N * N;
+        float2 *dst = a2 + ((size_t)b * NG + x) * N * N;
         if (col)
             sfor<0, N2>([&](auto kc) {
                 constexpr int k2 = decltype(kc)::value;
-                dst[(size_t)(u + N1 * k2) * N + kz] = y[k2];
+                st_at(dst, 8u * (unsigned)((u + N1 * k2) * N + kz), y[k2]);
             });
     }
 }
@@ -86,8 +103,9 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
 template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
-              const float2 *__restrict__ twg, int ng, int nd, int H, int Hp, int B) {
+              const float2 *__restrict__ twg, int H, int Hp, int B) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
+    constexpr int NG = N / 2, ND = N / 4;
     constexpr int L2 = 2 * L, LS2 = L2 + 1, LS = L + 1, NT = L2 * T;
     constexpr int PER = (L * N + NT - 1) / NT;  // fold elements per thread
     __shared__ float2 S[N * LS2];
@@ -106,8 +124,8 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
             const int l = q % L, kx = q / L, kz = c * L + l;
             if (kz < H) {
                 const int kxp = (N - kx) % N, kzp = (N - kz) % N;
-                a = rec[((size_t)kx * N + ky) * N + kz];
-                d = rec[((size_t)kxp * N + kyp) * N + kzp];
+                a = ld_at(rec, 8u * (unsigned)((kx * N + ky) * N + kz));
+                d = ld_at(rec, 8u * (unsigned)((kxp * N + kyp) * N + kzp));
             }
         }
     };
@@ -125,14 +143,15 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     for (int bb = 0; bb < RB; bb++) {
         const int b = b0 + bb;
         if (b >= B) break;  // uniform across the block
-        const float2 *src = a2 + (size_t)b * ng * N * N;
+        const float2 *src = a2 + (size_t)b * NG * N * N;  // uniform base, 32-bit lane offsets
         __syncthreads();  // S, Q free (previous rotation done)
         if (u < N2) {
             float2 v[N1];
+            const unsigned o0 = 8u * (unsigned)((u - ND) * N * N + kyl * N + kzs);  // used only when x is in range
             sfor<0, N1>([&](auto jc) {
                 constexpr int j1 = decltype(jc)::value;
-                const int x = u + N2 * j1 - nd;
-                v[j1] = (valid_line && x >= 0 && x < ng) ? src[((size_t)x * N + kyl) * N + kzs] : make_float2(0.f, 0.f);
+                const int x = u + N2 * j1 - ND;
+                v[j1] = (valid_line && x >= 0 && x < NG) ? ld_at(src, o0 + 8u * N2 * j1 * N * N) : make_float2(0.f, 0.f);
             });
             rf::step1_store<N, false>(v, S, tw, LS2, line, u);
         }
@@ -183,10 +202,11 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
             rf::step2_load<N, true>(z, S, LS, lq, uq);
             const int kz = c * L + lq;
             if (kz < Hp) {
-                float2 *dst = cc + (size_t)b * N * N * Hp;
+                float2 *dst = cc + (size_t)b * N * N * Hp;  // uniform base, 32-bit lane offsets
+                const unsigned o0 = 8u * (unsigned)((uq * N + ky) * Hp + kz);
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
-                    dst[((size_t)(uq + N1 * k2) * N + ky) * Hp + kz] = z[k2];
+                    st_at(dst, o0 + 8u * (unsigned)(N1 * k2 * N * Hp), z[k2]);
                 });
             }
         }
@@ -332,184 +352,193 @@ constexpr int plane_stride(int n) { return ((n / 2 + 1) % 2) ? n / 2 + 1 : n / 2
 constexpr bool fused_inverse(int n) { return (size_t)n * plane_stride(n) * 8 <= 150 * 1024; }
 
 // ------------------------------------------------------------------ fused inverse y + z + top-t
-// One workgroup per (x, b) plane of cc (nf rows ky, kz <= nf/2 columns) held in LDS:
-//   y-IFFT of every kz column in place;
+// Persistent workgroups, each walking the (b, x) planes p = b*nf + x of cc (nf rows ky,
+// kz <= nf/2 columns) with stride gridDim.x.  Per plane, held in LDS:
+//   y-IFFT of every kz column in place (step-1 inputs come straight from HBM into
+//   registers; the next plane's are loaded while the current one is computed);
 //   per row, the half-length complex-to-real step: with M = nf/2 and w = exp(+2 pi i / nf),
 //     Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])),  k < M,
 //   whose M-point inverse gives z[m] = x[2m] + i x[2m+1] (x the real score row);
 //   the M-point inverse of each row, whose outputs go straight into the top-t lists.
-// The plane's top-t is written as one candidate list per (x, b).
-template <int N, int TT>
+// The plane's top-t is written as one candidate list per plane.  PF: prefetch the next
+// plane's step-1 inputs (in flight during the current plane) or load them at the start.
+template <int N, int TT, bool PF>
 __global__ void __launch_bounds__(1024)
     kp_inv_yz_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
-                  const float2 *__restrict__ twg, int Hp, int t, float nf3f) {
+                  const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
     constexpr int NTH = 1024, NW = NTH / 64;
     constexpr int M = N / 2, H = M + 1;
     constexpr int LSP = plane_stride(N);  // row stride of the plane (odd)
     constexpr int N1y = Geo<N>::N1, N2y = Geo<N>::N2;
     constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2;
-    constexpr int KY = (H * N1y + NTH - 1) / NTH;  // y tasks per thread
     constexpr int KZ = (N * N1z + NTH - 1) / NTH;  // z tasks per thread
+    static_assert(H * N2y <= NTH && H * N1y <= NTH, "one y task per thread");
     __shared__ float2 P[N * LSP];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
     const int tid = threadIdx.x;
     for (int q = tid; q < N; q += NTH) tw[q] = twg[q];
-    const int x = blockIdx.x, b = blockIdx.y;
-    const float2 *src = cc + ((size_t)b * N + x) * N * Hp;
-    for (int q = tid; q < N * H; q += NTH) {
-        const int ky = q / H, kz = q - ky * H;
-        P[ky * LSP + kz] = src[(size_t)ky * Hp + kz];
-    }
-    __syncthreads();
-    // ---- y-IFFT of the H columns (line = kz, element = ky): element stride LSP
-    {
-        float2 v[KY][N1y];
-#pragma unroll
-        for (int j = 0; j < KY; j++) {
-            const int task = tid + j * NTH, line = task % H, u = task / H;
-            if (u < N2y)
-                sfor<0, N1y>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    v[j][j1] = P[(u + N2y * j1) * LSP + line];
-                });
+    // y task of a thread: column kz = yl, step-1 index yu (< N2y), step-2 index yu (< N1y)
+    float2 vn[N1y];  // step-1 inputs of the plane to come
+    auto load_plane = [&](int p, int yl, int yu) {
+        if (yu < N2y && p < nplanes) {
+            const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
+            const unsigned off = (unsigned)(yu * Hp + yl);
+            sfor<0, N1y>([&](auto jc) {
+                constexpr int j1 = decltype(jc)::value;
+                vn[j1] = src[off + (unsigned)(N2y * j1 * Hp)];
+            });
         }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < KY; j++) {
-            const int task = tid + j * NTH, line = task % H, u = task / H;
-            if (u < N2y) rf::step1_g<N, true>(v[j], P, tw, 1, LSP, 1, line, u);
+    };
+    if (PF) load_plane(blockIdx.x, tid % H, tid / H);
+    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
+        const int b = p / N, x = p - b * N;
+        // The thread's LDS addresses are recomputed in every iteration: an opaque copy of
+        // the thread index keeps them from being hoisted out of the plane loop, where they
+        // would all stay live and spill.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int yl = tid % H, yu = tid / H;
+        if (!PF) load_plane(p, yl, yu);
+        __syncthreads();  // P, tw and wl are free: the previous plane is done with them
+        // ---- y-IFFT of the H columns (line = kz, element = ky): element stride LSP
+        if (yu < N2y) {
+            float2 v[N1y];
+            sfor<0, N1y>([&](auto jc) { v[decltype(jc)::value] = vn[decltype(jc)::value]; });
+            rf::step1_g<N, true>(v, P, tw, 1, LSP, 1, yl, yu);
         }
+        if (PF) load_plane(p + gridDim.x, yl, yu);
         __syncthreads();
-        float2 y[KY][N2y];
-#pragma unroll
-        for (int j = 0; j < KY; j++) {
-            const int task = tid + j * NTH, line = task % H, u = task / H;
-            if (u < N1y) rf::step2_g<N, true>(y[j], P, LSP, 1, line, u);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < KY; j++) {
-            const int task = tid + j * NTH, line = task % H, u = task / H;
-            if (u < N1y)
+        {
+            float2 y[N2y];
+            if (yu < N1y) rf::step2_g<N, true>(y, P, LSP, 1, yl, yu);
+            __syncthreads();
+            if (yu < N1y)
                 sfor<0, N2y>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
-                    P[(u + N1y * k2) * LSP + line] = y[j][k2];
-                });
-        }
-    }
-    __syncthreads();
-    // ---- complex-to-real pre-step on each row, pairs (k, M-k) for k <= M/2
-    for (int q = tid; q < N * (M / 2 + 1); q += NTH) {
-        const int row = q / (M / 2 + 1), k = q - row * (M / 2 + 1);
-        float2 *R = P + row * LSP;
-        const float2 a = R[k], c = R[M - k];
-        // Z[k] from (X[k], X[M-k]); w^k = conj(tw[k])
-        const float2 wk = make_float2(tw[k].x, -tw[k].y);
-        const float2 s1 = make_float2(a.x + c.x, a.y - c.y);     // X[k] + conj(X[M-k])
-        const float2 d1 = make_float2(a.x - c.x, a.y + c.y);     // X[k] - conj(X[M-k])
-        const float2 e1 = make_float2(wk.x * d1.x - wk.y * d1.y, wk.x * d1.y + wk.y * d1.x);
-        const float2 z1 = make_float2(s1.x - e1.y, s1.y + e1.x);  // s1 + i e1
-        if (k == 0) {
-            R[0] = z1;
-        } else {
-            const int k2 = M - k;  // Z[M-k] from (X[M-k], X[k]); w^(M-k) = conj(tw[M-k])
-            const float2 w2 = make_float2(tw[k2].x, -tw[k2].y);
-            const float2 s2 = make_float2(c.x + a.x, c.y - a.y);
-            const float2 d2 = make_float2(c.x - a.x, c.y + a.y);
-            const float2 e2 = make_float2(w2.x * d2.x - w2.y * d2.y, w2.x * d2.y + w2.y * d2.x);
-            R[k] = z1;
-            R[k2] = make_float2(s2.x - e2.y, s2.y + e2.x);
-        }
-    }
-    __syncthreads();
-    // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
-    float2 y[KZ][N2z];
-    {
-        float2 v[KZ][N1z];
-#pragma unroll
-        for (int j = 0; j < KZ; j++) {
-            const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N2z)
-                sfor<0, N1z>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    v[j][j1] = P[line * LSP + u + N2z * j1];
+                    P[(yu + N1y * k2) * LSP + yl] = y[k2];
                 });
         }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < KZ; j++) {
-            const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
+        // ---- complex-to-real pre-step on each row, pairs (k, M-k) for k <= M/2
+        for (int q = tid; q < N * (M / 2 + 1); q += NTH) {
+            const int row = q / (M / 2 + 1), k = q - row * (M / 2 + 1);
+            float2 *R = P + row * LSP;
+            const float2 a = R[k], c = R[M - k];
+            // Z[k] from (X[k], X[M-k]); w^k = conj(tw[k])
+            const float2 wk = make_float2(tw[k].x, -tw[k].y);
+            const float2 s1 = make_float2(a.x + c.x, a.y - c.y);     // X[k] + conj(X[M-k])
+            const float2 d1 = make_float2(a.x - c.x, a.y + c.y);     // X[k] - conj(X[M-k])
+            const float2 e1 = make_float2(wk.x * d1.x - wk.y * d1.y, wk.x * d1.y + wk.y * d1.x);
+            const float2 z1 = make_float2(s1.x - e1.y, s1.y + e1.x);  // s1 + i e1
+            if (k == 0) {
+                R[0] = z1;
+            } else {
+                const int k2 = M - k;  // Z[M-k] from (X[M-k], X[k]); w^(M-k) = conj(tw[M-k])
+                const float2 w2 = make_float2(tw[k2].x, -tw[k2].y);
+                const float2 s2 = make_float2(c.x + a.x, c.y - a.y);
+                const float2 d2 = make_float2(c.x - a.x, c.y + a.y);
+                const float2 e2 = make_float2(w2.x * d2.x - w2.y * d2.y, w2.x * d2.y + w2.y * d2.x);
+                R[k] = z1;
+                R[k2] = make_float2(s2.x - e2.y, s2.y + e2.x);
+            }
         }
         __syncthreads();
+        // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
+        // zero-initialised so that no value of the arrays is carried around the plane loop
+        float2 y[KZ][N2z] = {};
+        {
+            float2 v[KZ][N1z] = {};
 #pragma unroll
-        for (int j = 0; j < KZ; j++) {
-            const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N1z) rf::step2_g<M, true>(y[j], P, 1, LSP, line, u);
+            for (int j = 0; j < KZ; j++) {
+                const int task = tid + j * NTH, line = task % N, u = task / N;
+                if (u < N2z)
+                    sfor<0, N1z>([&](auto jc) {
+                        constexpr int j1 = decltype(jc)::value;
+                        v[j][j1] = P[line * LSP + u + N2z * j1];
+                    });
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < KZ; j++) {
+                const int task = tid + j * NTH, line = task % N, u = task / N;
+                if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < KZ; j++) {
+                const int task = tid + j * NTH, line = task % N, u = task / N;
+                if (u < N1z) rf::step2_g<M, true>(y[j], P, 1, LSP, line, u);
+            }
         }
-    }
-    // z = 2m and 2m+1 of row `line` are y[j][k2].x and .y with m = u + N1z k2
-    if (smap) {  // uniform
+        // z = 2m and 2m+1 of row `line` are y[j][k2].x and .y with m = u + N1z k2
+        if (smap) {  // uniform
+#pragma unroll
+            for (int j = 0; j < KZ; j++) {
+                const int task = tid + j * NTH, line = task % N, u = task / N;
+                if (u < N1z)
+                    sfor<0, N2z>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const size_t o = (size_t)b * N * N * N + (size_t)(x * N + line) * N + 2 * (u + N1z * k2);
+                        smap[o] = y[j][k2].x;
+                        smap[o + 1] = y[j][k2].y;
+                    });
+            }
+            continue;
+        }
+        // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
+        // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
+        // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
+        constexpr bool QUOT = TT > 1;
+        float mx = -INFINITY;
 #pragma unroll
         for (int j = 0; j < KZ; j++) {
-            const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N1z)
+            if ((tid + j * NTH) / N < N1z)
                 sfor<0, N2z>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
-                    const size_t o = (size_t)b * N * N * N + (size_t)(x * N + line) * N + 2 * (u + N1z * k2);
-                    smap[o] = y[j][k2].x;
-                    smap[o + 1] = y[j][k2].y;
+                    mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
                 });
         }
-        return;
-    }
-    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
-    // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
-    // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
-    constexpr bool QUOT = TT > 1;
-    float mx = -INFINITY;
+        const float tau = wave_kth_max(mx, t);
+        const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
+        TopR<TT> tl;
+        tl.init();
 #pragma unroll
-    for (int j = 0; j < KZ; j++) {
-        if ((tid + j * NTH) / N < N1z)
-            sfor<0, N2z>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
-            });
-    }
-    const float tau = wave_kth_max(mx, t);
-    const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
-    TopR<TT> tl;
-    tl.init();
-#pragma unroll
-    for (int j = 0; j < KZ; j++) {
-        const int task = tid + j * NTH, line = task % N, u = task / N;
-        if (u < N1z) {
-            const int base = (x * N + line) * N + 2 * u;
-            sfor<0, N2z>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                const float v0 = y[j][k2].x, v1 = y[j][k2].y;
-                if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, base + 2 * N1z * k2);
-                if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, base + 2 * N1z * k2 + 1);
-            });
+        for (int j = 0; j < KZ; j++) {
+            const int task = tid + j * NTH, line = task % N, u = task / N;
+            if (u < N1z) {
+                const int base = (x * N + line) * N + 2 * u;
+                sfor<0, N2z>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const float v0 = y[j][k2].x, v1 = y[j][k2].y;
+                    if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, base + 2 * N1z * k2);
+                    if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, base + 2 * N1z * k2 + 1);
+                });
+            }
         }
+        block_top<TT, NW>(tl, t, wl, cand + (size_t)p * t);
     }
-    block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * N + x) * t);
 }
 
 template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
-    kp_inv_yz_top<N, TT><<<dim3(N, a.B), 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f);
+    const int nplanes = a.B * N;
+    const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
+    if (a.inv_variant == 1)
+        kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    else
+        kp_inv_yz_top<N, TT, false><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
 }
 
 template <int N> hipError_t run_pipe(const PipeArgs &a) {
     constexpr int T = Geo<N>::T;
     constexpr int L = 16, LX = 8;
-    const int B = a.B, ng = a.ng, nd = a.nd;
+    const int B = a.B, ng = a.ng;
+    if (ng != N / 2 || a.nd != N / 4) return hipErrorInvalidValue;  // the kernels assume nf = 2 ng, nd = ng / 2
     hipStream_t s = a.stream;
     const int lines_z = B * ng * ng;
-    kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.lg, a.a1, a.tw4, ng, nd, lines_z);
+    kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.vox, a.a1, a.tw4, lines_z);
     if (a.ev) (void)hipEventRecord(a.ev[0], s);
-    kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4, ng, nd);
+    kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
     // the grid, so consecutive workgroups share the receptor tile in L2 (3, the default);
@@ -518,19 +547,17 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     switch (a.variant) {
         case 0:
             kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
         case 1:
             kp_x_corr<N, LX, 4, false><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, ng, nd, a.H, a.Hp, B);
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
         case 2:
-            kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng, nd,
-                                                                                   a.H, a.Hp, B);
+            kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
         default:
-            kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, ng,
-                                                                                        nd, a.H, a.Hp, B);
+            kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
     }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     if constexpr (fused_inverse(N)) {

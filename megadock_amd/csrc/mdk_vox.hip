@@ -14,6 +14,13 @@
 //               again non-cascading because delta != 0 and 1 != 0)
 //   grid_i[v] = charges of the atoms nearest to v, summed in atom order (ligand.cpp:212-235)
 // delta == 0 makes the last rule order-dependent; the host rejects it.
+//
+// Representation handed to the FFT passes (VoxGrid): per rotation, a bit mask of the
+// voxels whose shape value is delta (D) and of those whose value is 1 (O), each row
+// (i, j) of ng voxels in its own rw = ceil(ng/32) words, and a dense float charge grid.
+// The rules above become word-wide bit operations over rows; the charge grid and the
+// per-voxel atom counts are kept all-zero between batches (their consumers clear what
+// they read), so a batch writes only the voxels its atoms touch.
 #pragma clang fp contract(off)
 
 #include "mdk_internal.h"
@@ -36,24 +43,21 @@ __global__ void k_clear(uint32_t *p, size_t n) {
         p[i] = 0u;
 }
 
-// set bits [k0, k1) of the row starting at voxel v0 (k1 - k0 <= 32)
-__device__ __forceinline__ void or_run(uint32_t *m, int v0, int k0, int k1) {
+// set bits [k0, k1) of a row that starts at word `row` (k1 - k0 <= 5)
+__device__ __forceinline__ void or_run(uint32_t *m, int row, int k0, int k1) {
     if (k1 <= k0) return;
-    const int a = v0 + k0, e = v0 + k1;  // [a, e)
-    const int w0 = a >> 5, w1 = (e - 1) >> 5;
+    const int w0 = k0 >> 5, w1 = (k1 - 1) >> 5;
     if (w0 == w1) {
-        const uint32_t n = (uint32_t)(e - a);
-        const uint32_t mask = (n == 32 ? 0xffffffffu : ((1u << n) - 1u)) << (a & 31);
-        atomicOr(&m[w0], mask);
+        atomicOr(&m[row + w0], ((1u << (k1 - k0)) - 1u) << (k0 & 31));
     } else {
-        atomicOr(&m[w0], 0xffffffffu << (a & 31));
-        atomicOr(&m[w1], 0xffffffffu >> (31 - ((e - 1) & 31)));
+        atomicOr(&m[row + w0], 0xffffffffu << (k0 & 31));
+        atomicOr(&m[row + w1], 0xffffffffu >> (31 - ((k1 - 1) & 31)));
     }
 }
 
 // one thread per (atom, rotation): core/surface bits, the charge voxel and its atom count
-__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t *core,
-                            uint32_t *surf, uint32_t *cnt, int *vidx, int nw, int cw) {
+__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t *cs, uint32_t *cnt, int *vidx,
+                            int mw, int cw) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (l >= L.na) return;
@@ -62,7 +66,7 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t
     for (int q = 0; q < 9; q++) r[q] = rot[9 * b + q];
     float X, Y, Z;
     rotate_atom(L, r, l, X, Y, Z);
-    const int ng = L.ng;
+    const int ng = L.ng, rw = row_words(ng);
     const float h = (float)(ng / 2);
     const int i2 = (int)(X / L.gw + h);
     const int j2 = (int)(Y / L.gw + h);
@@ -70,8 +74,8 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t
     const int ia = max(i2 - 2, 0), ja = max(j2 - 2, 0), ka = max(k2 - 2, 0);
     const int ib = min(i2 + 3, ng), jb = min(j2 + 3, ng), kb = min(k2 + 3, ng);
     const float rc2 = L.rc2[l], rs2 = L.rs2[l];
-    uint32_t *cb = core + (size_t)b * nw;
-    uint32_t *sb = surf + (size_t)b * nw;
+    uint32_t *cb = cs + (size_t)b * 2 * mw;  // core bits
+    uint32_t *sb = cb + mw;                  // surface bits
     for (int i = ia; i < ib; i++) {
         const float dx = X - L.gc[i];
         const float xd = dx * dx;
@@ -89,9 +93,9 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t
                 if (d3 < rc2) { c0 = min(c0, k); c1 = k + 1; }
                 if (d3 < rs2) { s0 = min(s0, k); s1 = k + 1; }
             }
-            const int v0 = (i * ng + j) * ng;
-            or_run(cb, v0, c0, c1);
-            or_run(sb, v0, s0, s1);
+            const int row = (i * ng + j) * rw;
+            or_run(cb, row, c0, c1);
+            or_run(sb, row, s0, s1);
         }
     }
     const int ci = max(0, min(ng - 1, (int)((X + L.pad) / L.gw)));
@@ -102,82 +106,153 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t
     atomicAdd(&cnt[(size_t)b * cw + (v >> 1)], 1u << (16 * (v & 1)));
 }
 
-__device__ __forceinline__ bool bit(const uint32_t *m, int v) { return (m[v >> 5] >> (v & 31)) & 1u; }
-
-// core after the scrape step
-__device__ __forceinline__ bool core2(const uint32_t *c, int ng, int i, int j, int k) {
-    const int v = (i * ng + j) * ng + k;
-    if (!bit(c, v)) return false;
-    const bool interior = i >= 1 && i <= ng - 2 && j >= 1 && j <= ng - 2 && k >= 1 && k <= ng - 2;
-    if (!interior) return true;
-    const int ng2 = ng * ng;
-    const bool open = !bit(c, v - 1) || !bit(c, v + 1) || !bit(c, v - ng) || !bit(c, v + ng) ||
-                      !bit(c, v - ng2) || !bit(c, v + ng2);
-    return !open;
+// bits k = 32 w + s of word w with lo <= k <= hi
+__device__ __forceinline__ uint32_t range_bits(int w, int lo, int hi) {
+    const int a = max(lo - 32 * w, 0), e = min(hi - 32 * w, 31);
+    if (a > e) return 0u;
+    const uint32_t upto = (e == 31) ? 0xffffffffu : ((1u << (e + 1)) - 1u);
+    return upto & (0xffffffffu << a);
 }
 
-__global__ void k_vox_grid(const uint32_t *core, const uint32_t *surf, float2 *lg, int ng, int nw,
-                           float delta) {
-    const int ng3 = ng * ng * ng;
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (v >= ng3) return;
-    const uint32_t *c = core + (size_t)b * nw;
-    const uint32_t *s = surf + (size_t)b * nw;
-    const int i = v / (ng * ng), j = (v / ng) % ng, k = v % ng;
-    const bool cc = core2(c, ng, i, j, k);
-    float val = cc ? delta : (bit(s, v) ? 1.0f : 0.0f);
-    if (cc && delta != 1.0f && i >= 1 && i <= ng - 2 && j >= 1 && j <= ng - 2 && k >= 1 &&
-        k <= ng - 2) {
-        const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
-        bool zero_nb = false;
-        for (int q = 0; q < 6; q++) {
-            const int a = i + di[q], bb = j + dj[q], cq = k + dk[q];
-            const int w = (a * ng + bb) * ng + cq;
-            if (!core2(c, ng, a, bb, cq) && !bit(s, w)) zero_nb = true;
-        }
-        if (zero_nb) val = 1.0f;
+struct MaskRows {
+    const uint32_t *m;
+    int ng, rw;
+    // word w of row (i, j); zero outside the grid
+    __device__ __forceinline__ uint32_t at(int i, int j, int w) const {
+        if (i < 0 || i >= ng || j < 0 || j >= ng || w < 0 || w >= rw) return 0u;
+        return m[(i * ng + j) * rw + w];
     }
-    lg[(size_t)b * ng3 + v] = make_float2(val, 0.0f);
+};
+
+// core after the scrape step, word w of row (i, j)
+__device__ __forceinline__ uint32_t core2_word(const MaskRows &C, int i, int j, int w) {
+    const int ng = C.ng;
+    const uint32_t c = C.at(i, j, w);
+    if (i < 1 || i > ng - 2 || j < 1 || j > ng - 2 || c == 0u) return c;
+    const uint32_t zm = (c << 1) | (C.at(i, j, w - 1) >> 31);  // bit k: voxel k-1 is core
+    const uint32_t zp = (c >> 1) | (C.at(i, j, w + 1) << 31);  // bit k: voxel k+1 is core
+    const uint32_t closed = zm & zp & C.at(i, j - 1, w) & C.at(i, j + 1, w) & C.at(i - 1, j, w) & C.at(i + 1, j, w);
+    return c & ~(range_bits(w, 1, ng - 2) & ~closed);
+}
+
+// one thread per (row word, rotation): the delta mask D and the value-1 mask O
+__global__ void k_vox_masks(const uint32_t *__restrict__ cs, uint32_t *dmo, int ng, int mw, float delta) {
+    const int rw = row_words(ng);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (q >= mw) return;
+    const int w = q % rw, row = q / rw, i = row / ng, j = row - i * ng;
+    const MaskRows C{cs + (size_t)b * 2 * mw, ng, rw}, S{cs + (size_t)b * 2 * mw + mw, ng, rw};
+    uint32_t d = core2_word(C, i, j, w);
+    uint32_t o = ~d & S.at(i, j, w);
+    if (delta != 1.0f && i >= 1 && i <= ng - 2 && j >= 1 && j <= ng - 2) {
+        // interior delta voxels next to a voxel of value 0 (neither core nor surface) become 1
+        auto zero = [&](int ii, int jj, int ww) { return ~core2_word(C, ii, jj, ww) & ~S.at(ii, jj, ww); };
+        const uint32_t z0 = zero(i, j, w);
+        const uint32_t zm = (z0 << 1) | (zero(i, j, w - 1) >> 31);
+        const uint32_t zp = (z0 >> 1) | (zero(i, j, w + 1) << 31);
+        const uint32_t nb = zm | zp | zero(i, j - 1, w) | zero(i, j + 1, w) | zero(i - 1, j, w) | zero(i + 1, j, w);
+        const uint32_t to1 = d & range_bits(w, 1, ng - 2) & nb;
+        d &= ~to1;
+        o |= to1;
+    }
+    dmo[(size_t)b * 2 * mw + q] = d;
+    dmo[(size_t)b * 2 * mw + mw + q] = o;
 }
 
 // Charges of the atoms nearest to each voxel, summed in atom order from +0.0f
-// (src/ligand.cpp:216-234).  With one or two atoms in a voxel the float sum does not
-// depend on the order (0 + a = a and a + b = b + a exactly), so those are added
-// atomically; a voxel holding three or more is summed in atom order by its
-// lowest-index atom, which is then its only writer.
-__global__ void k_vox_charge(const int *__restrict__ vidx, const uint32_t *__restrict__ cnt,
-                             const float *__restrict__ charge, float2 *lg, int na, int ng3, int cw) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (l >= na) return;
+// (src/ligand.cpp:216-234).  One workgroup per rotation.  With one or two atoms in a
+// voxel the float sum does not depend on the order (0 + a = a and a + b = b + a
+// exactly), so those are added atomically; atoms of voxels holding three or more are
+// listed in LDS and each such voxel is summed in atom order by its lowest-index atom.
+// Leaves the rotation's atom counts zero again.
+constexpr int kChargeThreads = 1024;
+constexpr int kCrowdCap = 2048;  // listed atoms of crowded voxels (beyond: global scan)
+
+__global__ void __launch_bounds__(kChargeThreads)
+    k_vox_charge(const int *__restrict__ vidx, uint32_t *cnt, const float *__restrict__ charge, float *chg,
+                 int na, int ng3, int cw) {
+    __shared__ int nlist;
+    __shared__ int2 list[kCrowdCap];  // (voxel, atom)
+    const int b = blockIdx.x, tid = threadIdx.x;
     const int *vi = vidx + (size_t)b * na;
-    const int v = vi[l];
-    const uint32_t n = (cnt[(size_t)b * cw + (v >> 1)] >> (16 * (v & 1))) & 0xffffu;
-    float *dst = &lg[(size_t)b * ng3 + v].y;
-    if (n <= 2) {
-        atomicAdd(dst, charge[l]);
-        return;
+    uint32_t *cb = cnt + (size_t)b * cw;
+    float *g = chg + (size_t)b * ng3;
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    for (int l = tid; l < na; l += kChargeThreads) {
+        const int v = vi[l];
+        const uint32_t n = (cb[v >> 1] >> (16 * (v & 1))) & 0xffffu;
+        if (n <= 2) {
+            atomicAdd(&g[v], charge[l]);
+        } else {
+            const int k = atomicAdd(&nlist, 1);
+            if (k < kCrowdCap) list[k] = make_int2(v, l);
+        }
     }
-    for (int q = 0; q < l; q++)
-        if (vi[q] == v) return;
-    float sum = 0.0f;
-    for (int q = l; q < na; q++)
-        if (vi[q] == v) sum += charge[q];
-    *dst = sum;
+    __syncthreads();
+    const int m = nlist;  // uniform
+    if (m <= kCrowdCap) {
+        for (int e = tid; e < m; e += kChargeThreads) {
+            const int v = list[e].x, l = list[e].y;
+            bool owner = true;
+            for (int f = 0; f < m; f++)
+                if (list[f].x == v && list[f].y < l) owner = false;
+            if (!owner) continue;
+            float sum = 0.0f;
+            for (int cur = -1;;) {  // next atom of the voxel in index order
+                int nxt = 0x7fffffff;
+                for (int f = 0; f < m; f++)
+                    if (list[f].x == v && list[f].y > cur && list[f].y < nxt) nxt = list[f].y;
+                if (nxt == 0x7fffffff) break;
+                sum += charge[nxt];
+                cur = nxt;
+            }
+            g[v] = sum;
+        }
+    } else {
+        for (int l = tid; l < na; l += kChargeThreads) {
+            const int v = vi[l];
+            const uint32_t n = (cb[v >> 1] >> (16 * (v & 1))) & 0xffffu;
+            if (n <= 2) continue;
+            bool owner = true;
+            for (int q = 0; q < l; q++)
+                if (vi[q] == v) owner = false;
+            if (!owner) continue;
+            float sum = 0.0f;
+            for (int q = l; q < na; q++)
+                if (vi[q] == v) sum += charge[q];
+            g[v] = sum;
+        }
+    }
+    __syncthreads();  // every count read above is done
+    for (int l = tid; l < na; l += kChargeThreads) cb[vi[l] >> 1] = 0u;
+}
+
+// grids as (grid_r, grid_i) pairs; clears the charge grid it reads
+__global__ void k_vox_expand(VoxGrid vg, float2 *lg, int ng3) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (v >= ng3) return;
+    lg[(size_t)b * ng3 + v] = vox_take(vg, b, v / vg.ng, v % vg.ng);
 }
 
 }  // namespace
 
-hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, uint32_t *bits, int *vidx,
-                           float2 *lg, hipStream_t s) {
-    const int nw = mask_words(lig.ng), cw = count_words(lig.ng);
-    const size_t nclear = (size_t)B * (2 * (size_t)nw + cw);
-    uint32_t *core = bits, *surf = bits + (size_t)B * nw, *cnt = bits + 2 * (size_t)B * nw;
-    const int ng3 = lig.ng * lig.ng * lig.ng;
-    k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(bits, nclear);
-    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, core, surf, cnt, vidx, nw, cw);
-    k_vox_grid<<<dim3((ng3 + 255) / 256, B), 256, 0, s>>>(core, surf, lg, lig.ng, nw, lig.delta);
-    k_vox_charge<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(vidx, cnt, lig.charge, lg, lig.na, ng3, cw);
+hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const VoxScratch &vs, int *vidx,
+                           float *chg, hipStream_t s) {
+    const int ng = lig.ng, mw = mask_words(ng), cw = count_words(ng);
+    const int ng3 = ng * ng * ng;
+    const size_t nclear = 2 * (size_t)B * mw;  // core and surface bits; counts are kept zero
+    k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(vs.cs, nclear);
+    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, vs.cs, vs.cnt, vidx, mw, cw);
+    k_vox_masks<<<dim3((mw + 255) / 256, B), 256, 0, s>>>(vs.cs, vs.dmo, ng, mw, lig.delta);
+    k_vox_charge<<<B, kChargeThreads, 0, s>>>(vidx, vs.cnt, lig.charge, chg, lig.na, ng3, cw);
+    return hipGetLastError();
+}
+
+hipError_t launch_vox_expand(const VoxGrid &vg, int B, float2 *lg, hipStream_t s) {
+    const int ng3 = vg.ng * vg.ng * vg.ng;
+    k_vox_expand<<<dim3((ng3 + 255) / 256, B), 256, 0, s>>>(vg, lg, ng3);
     return hipGetLastError();
 }
