@@ -386,7 +386,7 @@ struct mdk_ctx {
     float2 *d_tw = nullptr;
     float2 *d_tw4 = nullptr;  // exp(-2 pi i m / nf): twiddles of the specialised pipeline
     bool use_pipe = false;    // compile-time specialised kernels exist for nf
-    int xcorr_variant = 3;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
+    int xcorr_variant = 4;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
     int cu_count = 256;       // compute units (persistent grids)
     int inv_variant = 0;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
     float2 *d_rec = nullptr;
@@ -587,7 +587,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         const char *g = std::getenv("MDK_FORCE_GENERIC");
         c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
-        c->xcorr_variant = xv ? std::atoi(xv) : 3;
+        c->xcorr_variant = xv ? std::atoi(xv) : 4;
         const char *iv = std::getenv("MDK_INV_VARIANT");
         c->inv_variant = iv ? std::atoi(iv) : 0;
         int cus = 0;

@@ -50,8 +50,8 @@ struct VoxGrid {
 };
 __device__ __forceinline__ float2 vox_take(const VoxGrid &g, int b, int row, int k) {
     const uint32_t *d = g.dmo + (size_t)b * 2 * g.mw + row * g.rw + (k >> 5);
-    const uint32_t bit = 1u << (k & 31);
-    const float sv = (d[0] & bit) ? g.delta : ((d[g.mw] & bit) ? 1.0f : 0.0f);
+    const uint32_t bit = 1u << (k & 31), dw = d[0], ow = d[g.mw];
+    const float sv = (dw & bit) ? g.delta : ((ow & bit) ? 1.0f : 0.0f);
     float *cp = g.chg + (size_t)b * g.ng3 + (size_t)row * g.ng + k;
     const float q = *cp;
     if (__float_as_uint(q) != 0u) *cp = 0.0f;

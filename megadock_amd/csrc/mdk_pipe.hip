@@ -27,32 +27,40 @@ template <class T> __device__ __forceinline__ void st_at(T *base, unsigned byte_
 
 // ------------------------------------------------------------------ forward z
 // lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1;
-// their values come straight from the voxel masks and the charge grid
+// their values come straight from the voxel masks and the charge grid into the step-1
+// registers.  Thread (line, u) with u fastest, so a line's voxels go to neighbouring lanes.
 template <int N, int L>
 __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(VoxGrid vg, float2 *__restrict__ a1,
                                                           const float2 *__restrict__ twg, int nlines) {
-    constexpr int T = Geo<N>::T, LS = L + 1, NT = L * T;
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
     constexpr int NG = N / 2, ND = N / 4;  // ligand voxels per axis and their offset in the line
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
-    const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    const int tid = threadIdx.x, line = tid / T, u = tid - line * T;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int line0 = blockIdx.x * L;
-    for (int q = tid; q < L * N; q += NT) {
-        const int l = q / N, e = q % N;
-        float2 v = make_float2(0.f, 0.f);
-        if (line0 + l < nlines && e >= ND && e < ND + NG) {
-            const int b = (line0 + l) / (NG * NG), row = line0 + l - b * NG * NG;
-            v = vox_take(vg, b, row, e - ND);
-        }
-        S[e * LS + l] = v;
+    const int gl = blockIdx.x * L + line;
+    const bool act = gl < nlines;
+    const int b = gl / (NG * NG), row = gl - b * (NG * NG);
+    __syncthreads();
+    if (u < N2) {
+        float2 v[N1];
+        sfor<0, N1>([&](auto jc) {
+            constexpr int j1 = decltype(jc)::value;
+            const int e = u + N2 * j1;
+            v[j1] = (act && e >= ND && e < ND + NG) ? vox_take(vg, b, row, e - ND) : make_float2(0.f, 0.f);
+        });
+        rf::step1_store<N, false>(v, S, tw, LS, line, u);
     }
     __syncthreads();
-    rf::four_step_inplace<N, L, false>(S, tw, LS, line, u, true);
-    float2 *dst = a1 + (size_t)line0 * N;  // uniform base, 32-bit lane offsets
-    for (int q = tid; q < L * N; q += NT) {
-        const int l = q / N, e = q % N;
-        if (line0 + l < nlines) st_at(dst, 8u * (unsigned)q, S[e * LS + l]);
+    if (u < N1) {
+        float2 y[N2];
+        rf::step2_load<N, false>(y, S, LS, line, u);
+        float2 *dst = a1 + (size_t)blockIdx.x * L * N;  // uniform base, 32-bit lane offsets
+        if (act)
+            sfor<0, N2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                st_at(dst, 8u * (unsigned)(line * N + u + N1 * k2), y[k2]);
+            });
     }
 }
 
@@ -100,7 +108,8 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
 // The receptor values of the tile are loaded once and reused for RB rotations.
 // B_FAST: the rotation is the fastest grid index, so the workgroups that share a
 // receptor tile run back to back and find it in L2 / the last-level cache.
-template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false>
+// QA: the folded lines Q share the LDS of S (two more barriers, a third less LDS).
+template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
               const float2 *__restrict__ twg, int H, int Hp, int B) {
@@ -109,8 +118,9 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     constexpr int L2 = 2 * L, LS2 = L2 + 1, LS = L + 1, NT = L2 * T;
     constexpr int PER = (L * N + NT - 1) / NT;  // fold elements per thread
     __shared__ float2 S[N * LS2];
-    __shared__ float2 Q[N * LS];
+    __shared__ float2 Qs[QA ? 1 : N * LS];
     __shared__ float2 tw[N];
+    float2 *const Q = QA ? S : Qs;
     const int tid = threadIdx.x;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
     const int c = B_FAST ? blockIdx.y : blockIdx.x, ky = B_FAST ? blockIdx.z : blockIdx.y;
@@ -166,6 +176,7 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
             });
         __syncthreads();
         // Hermitian fold into Q
+        float2 qv[PER];
 #pragma unroll
         for (int p = 0; p < PER; p++) {
             const int q = tid + p * NT;
@@ -181,20 +192,28 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
                 }
                 const float2 p1 = make_float2(a.x * l1.x + a.y * l1.y, a.x * l1.y - a.y * l1.x);
                 const float2 p2 = make_float2(d.x * l2.x + d.y * l2.y, d.x * l2.y - d.y * l2.x);
-                Q[kx * LS + l] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
+                qv[p] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
             }
+        }
+        if (QA) __syncthreads();  // every read of S above is done before Q overwrites it
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int q = tid + p * NT;
+            if (q < L * N) Q[(q / L) * LS + q % L] = qv[p];
         }
         __syncthreads();
         // inverse x-FFT of the L lines of Q; thread (lq, uq) over the first L*T threads
         const int lq = tid % L, uq = tid / L;
         const bool act = tid < L * T;
-        if (act && uq < N2) {
+        {
             float2 v[N1];
-            sfor<0, N1>([&](auto jc) {
-                constexpr int j1 = decltype(jc)::value;
-                v[j1] = Q[(uq + N2 * j1) * LS + lq];
-            });
-            rf::step1_store<N, true>(v, S, tw, LS, lq, uq);
+            if (act && uq < N2)
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    v[j1] = Q[(uq + N2 * j1) * LS + lq];
+                });
+            if (QA) __syncthreads();  // Q is read before step 1 overwrites it
+            if (act && uq < N2) rf::step1_store<N, true>(v, S, tw, LS, lq, uq);
         }
         __syncthreads();
         if (act && uq < N1) {
@@ -541,9 +560,10 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
-    // the grid, so consecutive workgroups share the receptor tile in L2 (3, the default);
-    // receptor tile cached in registers for 4 rotations (0); re-read from L2 for each of 4
-    // rotations (1); one rotation per workgroup, tile index fastest (2)
+    // the grid, so consecutive workgroups share the receptor tile in L2, folded lines kept
+    // in the exchange buffer's LDS (4, the default); the same with a separate fold buffer
+    // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
+    // of 4 rotations (1); one rotation per workgroup, tile index fastest (2)
     switch (a.variant) {
         case 0:
             kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
@@ -556,8 +576,12 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
         case 2:
             kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
-        default:
+        case 3:
             kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+            break;
+        default:
+            kp_x_corr<N, LX, 1, false, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
+                                                                                              a.H, a.Hp, B);
     }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     if constexpr (fused_inverse(N)) {

@@ -34,6 +34,22 @@ def kernel_stats(trace_dir):
     return rows
 
 
+def dispatch_stats(trace_dir):
+    """Per (kernel, grid) duration statistics from run_kernel_trace.csv: full batches and
+    single-rotation calls have different grids and are reported apart."""
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv"))):
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        d[(short_name(r["Kernel_Name"]), grid)].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    rows = []
+    for (k, g), v in d.items():
+        v.sort()
+        rows.append({"kernel": k, "grid": g, "calls": len(v), "avg_us": sum(v) / len(v) / 1e3,
+                     "median_us": v[len(v) // 2] / 1e3, "total_ms": sum(v) / 1e6})
+    rows.sort(key=lambda r: -r["total_ms"])
+    return rows
+
+
 def counter_per_kernel(pmc_dir, counter):
     vals = collections.defaultdict(list)
     for r in csv.DictReader(open(os.path.join(pmc_dir, "run_counter_collection.csv"))):
@@ -68,6 +84,20 @@ def traffic(fetch_dir, write_dir, batch, ff, wf, min_dispatches=4):
             "bytes_per_batch": total, "bytes_per_rotation": total / batch, "kernels": kernels}
 
 
+def counters(dirs, batch, kernels_like="kp_|k_vox|k_reduce|k_clear"):
+    """Counter values per full-batch dispatch for every counter found in the given pmc
+    output folders, divided by the batch (per rotation)."""
+    import re
+    table = collections.defaultdict(dict)
+    for d in dirs:
+        names = set(r["Counter_Name"] for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        for c in names:
+            for k, v in counter_per_kernel(d, c).items():
+                if re.match(kernels_like, k) and v["dispatches"] >= 4:
+                    table[k][c] = v["kib"] / batch  # the value per dispatch (named kib by the helper)
+    return table
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -80,11 +110,30 @@ def main(argv=None):
     f.add_argument("--copy-fetch-factor", type=float, default=2.0)
     f.add_argument("--copy-write-factor", type=float, default=1.0)
     f.add_argument("--json", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dirs", nargs="+")
+    c.add_argument("--batch", type=int, required=True)
     a = ap.parse_args(argv)
+    if a.cmd == "counters":
+        tab = counters(a.dirs, a.batch)
+        cols = sorted(set(x for v in tab.values() for x in v))
+        print("Per rotation (counter value of a full-batch dispatch / batch):\n")
+        print("| kernel | " + " | ".join(cols) + " |\n|---|" + "---:|" * len(cols))
+        for k in sorted(tab, key=lambda k: -tab[k].get("SQ_INSTS_VALU", 0)):
+            print("| %s | " % k + " | ".join("%.4g" % tab[k].get(x, float("nan")) for x in cols) + " |")
+        return 0
     if a.cmd == "trace":
+        print("rocprofv3 --stats summary (all dispatches):\n")
         print("| kernel | calls | avg (us) | total (ms) | % |\n|---|---:|---:|---:|---:|")
         for r in kernel_stats(a.dir):
             print("| %s | %d | %.1f | %.2f | %.2f |" % (r["kernel"], r["calls"], r["avg_us"], r["total_ms"], r["percent"]))
+        print("\nPer grid size, from the kernel trace (a full batch and a single rotation apart):\n")
+        print("| kernel | grid (work-items) | calls | avg (us) | median (us) | total (ms) |\n|---|---:|---:|---:|---:|---:|")
+        for r in dispatch_stats(a.dir):
+            if r["total_ms"] < 0.05:
+                continue
+            print("| %s | %d | %d | %.1f | %.1f | %.2f |" % (r["kernel"], r["grid"], r["calls"], r["avg_us"],
+                                                             r["median_us"], r["total_ms"]))
     else:
         s = traffic(a.fetch_dir, a.write_dir, a.batch, a.copy_fetch_factor, a.copy_write_factor)
         with open(a.json, "w") as fh:
