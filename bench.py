@@ -52,6 +52,34 @@ def stage_bytes(nf, ng, t):
     }
 
 
+def rocfft_reference(nf, batch, reps=5):
+    """torch.fft (hipFFT/rocFFT) forward + inverse 3D C2C of `batch` nf^3 complex64 grids: the
+    vendor FFT alone, as a reference point next to the whole rotation loop (which also
+    voxelizes, multiplies, folds and reduces)."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        x = torch.randn(batch, nf, nf, nf, dtype=torch.complex64, device="cuda")
+        for _ in range(2):
+            y = torch.fft.ifftn(torch.fft.fftn(x, dim=(1, 2, 3)), dim=(1, 2, 3))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            y = torch.fft.ifftn(torch.fft.fftn(x, dim=(1, 2, 3)), dim=(1, 2, 3))
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del x, y
+        torch.cuda.empty_cache()
+        return {"what": "forward + inverse 3D C2C FFT of %d grids of %d^3 complex64 (torch.fft on rocFFT)" % (batch, nf),
+                "ms": round(ms, 3), "grids_per_s": round(batch / (ms * 1e-3), 1),
+                "GBps_at_B_rot": round(batch * 32.0 * nf ** 3 / (ms * 1e-3) / 1e9, 1)}
+    except Exception as e:  # the reference point is optional
+        return {"error": str(e)[:200]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +90,7 @@ def main():
     ap.add_argument("--plain-rotations", type=int, default=200)
     ap.add_argument("--cpu-rotations", type=int, default=0, help="CPU baseline sample size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rocfft", action="store_true", help="skip the torch.fft (rocFFT) reference point")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
     ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i) and "
                     "batch sizes (b), e.g. 'v0,v3,i1,b64'; settings accumulate; prints one stage-timing line "
@@ -206,6 +235,9 @@ def main():
         "events_ms_per_pass": round(ev_ms, 2),
         "best_score": float(top["score"][0]),
     }
+
+    if not args.no_rocfft:
+        out["rocfft_reference"] = rocfft_reference(nf, min(B, 32))
 
     if not args.no_cpu:
         from oracle import oracle as O

@@ -28,6 +28,11 @@
 #include "mdk_internal.h"
 #include "../../include/mdk_hip.h"
 
+// device merge of the per-rotation results (mdk_merge.hip)
+size_t topn_scratch_bytes(int n);
+hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *scratch, mdk_sortscore *out,
+                       hipStream_t s);
+
 #define MDK_NSTAGE 7  // voxelize, fwd_z, fwd_y, x_corr, inv_y, inv_z_top, reduce
 
 namespace {
@@ -409,6 +414,9 @@ struct mdk_ctx {
     Cand *d_cand = nullptr;
     mdk_sortscore *d_bout = nullptr;   // max_batch * t
     mdk_sortscore *d_results = nullptr; // nrot * t
+    void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
+    mdk_sortscore *d_topn_out = nullptr;
+    int topn_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -614,7 +622,7 @@ int mdk_destroy(mdk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
                     c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_chg, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
-                    c->d_cc, c->d_cand, c->d_bout, c->d_results};
+                    c->d_cc, c->d_cand, c->d_bout, c->d_results, c->d_topn_scratch, c->d_topn_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -763,19 +771,23 @@ int mdk_get_results(mdk_ctx *c, int first, int count, mdk_sortscore *out) {
 int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
     if (!c || !out || !c->d_results) return fail("mdk_topn: nothing to merge");
     if (nrot < 1 || nrot > c->nrot || n < 1) return fail("mdk_topn: invalid count");
-    // merge on the host for now: ordering key (score desc, rotation asc, rank asc)
+    HIPCHK(hipSetDevice(c->device));
+    // on the device: order (score desc, rotation asc, rank asc), mdk_merge.hip
     const size_t m = (size_t)nrot * c->t;
-    std::vector<mdk_sortscore> all(m);
-    HIPCHK(hipMemcpyAsync(all.data(), c->d_results, sizeof(mdk_sortscore) * m, hipMemcpyDeviceToHost, c->stream));
+    const int keep = (int)std::min((size_t)n, m);
+    if (keep > c->topn_cap) {
+        if (c->d_topn_scratch) HIPCHK(hipFree(c->d_topn_scratch));
+        if (c->d_topn_out) HIPCHK(hipFree(c->d_topn_out));
+        c->d_topn_scratch = nullptr;
+        c->d_topn_out = nullptr;
+        c->topn_cap = 0;
+        HIPCHK(hipMalloc(&c->d_topn_scratch, topn_scratch_bytes(keep)));
+        HIPCHK(hipMalloc((void **)&c->d_topn_out, sizeof(mdk_sortscore) * keep));
+        c->topn_cap = keep;
+    }
+    HIPCHK(launch_topn(c->d_results, (unsigned)m, (unsigned)keep, c->d_topn_scratch, c->d_topn_out, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->d_topn_out, sizeof(mdk_sortscore) * keep, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<size_t> order(m);
-    for (size_t i = 0; i < m; i++) order[i] = i;
-    const size_t keep = std::min((size_t)n, m);
-    std::partial_sort(order.begin(), order.begin() + keep, order.end(), [&](size_t a, size_t b) {
-        if (all[a].score != all[b].score) return all[a].score > all[b].score;
-        return a < b;
-    });
-    for (size_t i = 0; i < keep; i++) out[i] = all[order[i]];
     return 0;
 }
 

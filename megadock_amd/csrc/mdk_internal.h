@@ -120,26 +120,51 @@ template <int TT> struct TopR {
     }
 };
 
-// Top-t of the sorted lists held by the 64 lanes of a wavefront, by t rounds of a
-// butterfly arg-max over the list heads; every lane receives the result in (ov, oi).
+// Wavefront-wide reductions with DPP lane moves inside each row of 16 lanes (quad
+// swaps, then row rotations by 4 and 8) and four lane reads across the rows; the result
+// is the same in every lane.  Every lane of the wavefront must take part.
+template <int CTRL> __device__ __forceinline__ int dpp_mov(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float wave_max(float v) {
+    v = fmaxf(v, __int_as_float(dpp_mov<0xb1>(__float_as_int(v))));   // quad_perm [1,0,3,2]
+    v = fmaxf(v, __int_as_float(dpp_mov<0x4e>(__float_as_int(v))));   // quad_perm [2,3,0,1]
+    v = fmaxf(v, __int_as_float(dpp_mov<0x124>(__float_as_int(v))));  // row_ror:4
+    v = fmaxf(v, __int_as_float(dpp_mov<0x128>(__float_as_int(v))));  // row_ror:8
+    const int b = __float_as_int(v);
+    return fmaxf(fmaxf(__int_as_float(__builtin_amdgcn_readlane(b, 0)), __int_as_float(__builtin_amdgcn_readlane(b, 16))),
+                 fmaxf(__int_as_float(__builtin_amdgcn_readlane(b, 32)), __int_as_float(__builtin_amdgcn_readlane(b, 48))));
+}
+__device__ __forceinline__ int wave_min(int v) {
+    v = min(v, dpp_mov<0xb1>(v));
+    v = min(v, dpp_mov<0x4e>(v));
+    v = min(v, dpp_mov<0x124>(v));
+    v = min(v, dpp_mov<0x128>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// Top-t of the sorted lists held by the 64 lanes of a wavefront, in t rounds: the best
+// head key (wave_max), the lowest index among the heads holding it, and a pop by the lane
+// that owns that entry.  Indices are unique except for the (-inf, kNoIndex) placeholders.
+// Every lane receives the result in (ov, oi).
 template <int TT, int R = 0>
 __device__ __forceinline__ void wave_top(TopR<TT> &L, int t, float *ov, int *oi) {
     if constexpr (R < TT) {  // one round per rank, unrolled at compile time
         if (R >= t) return;  // uniform
-        float bv = L.v[0];
-        int bi = L.i[0];
-#pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-            const float pv = __shfl_xor(bv, m);
-            const int pi = __shfl_xor(bi, m);
-            if (better(pv, pi, bv, bi)) {
-                bv = pv;
-                bi = pi;
-            }
-        }
-        ov[R] = bv;
-        oi[R] = bi;
-        if (L.v[0] == bv && L.i[0] == bi) L.pop();
+        const float m = wave_max(L.v[0]);
+        const bool at = L.v[0] == m;
+        const unsigned long long hs = __ballot(at);
+        int wi;
+        if (__popcll(hs) == 1)
+            wi = __builtin_amdgcn_readlane(L.i[0], __ffsll((long long)hs) - 1);
+        else
+            wi = wave_min(at ? L.i[0] : kNoIndex);
+        const bool own = at && L.i[0] == wi;
+        // the owner's own key (m may differ from it in the sign of a zero)
+        ov[R] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.v[0]), __ffsll((long long)__ballot(own)) - 1));
+        oi[R] = wi;
+        if (own) L.pop();
         wave_top<TT, R + 1>(L, t, ov, oi);
     }
 }
@@ -151,13 +176,7 @@ __device__ __forceinline__ float wave_kth_max(float m, int t) {
     float tau = -INFINITY;
     const int lane = (int)(threadIdx.x & 63);
     for (int r = 0; r < t; r++) {  // t is uniform
-        float w = m;
-        w = fmaxf(w, __shfl_xor(w, 32));
-        w = fmaxf(w, __shfl_xor(w, 16));
-        w = fmaxf(w, __shfl_xor(w, 8));
-        w = fmaxf(w, __shfl_xor(w, 4));
-        w = fmaxf(w, __shfl_xor(w, 2));
-        w = fmaxf(w, __shfl_xor(w, 1));
+        const float w = wave_max(m);
         tau = w;
         const unsigned long long holders = __ballot(m == w);
         if (holders && lane == __ffsll((long long)holders) - 1) m = -INFINITY;  // retire one holder

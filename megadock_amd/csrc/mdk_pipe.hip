@@ -439,31 +439,9 @@ __global__ void __launch_bounds__(1024)
                 });
         }
         __syncthreads();
-        // ---- complex-to-real pre-step on each row, pairs (k, M-k) for k <= M/2
-        for (int q = tid; q < N * (M / 2 + 1); q += NTH) {
-            const int row = q / (M / 2 + 1), k = q - row * (M / 2 + 1);
-            float2 *R = P + row * LSP;
-            const float2 a = R[k], c = R[M - k];
-            // Z[k] from (X[k], X[M-k]); w^k = conj(tw[k])
-            const float2 wk = make_float2(tw[k].x, -tw[k].y);
-            const float2 s1 = make_float2(a.x + c.x, a.y - c.y);     // X[k] + conj(X[M-k])
-            const float2 d1 = make_float2(a.x - c.x, a.y + c.y);     // X[k] - conj(X[M-k])
-            const float2 e1 = make_float2(wk.x * d1.x - wk.y * d1.y, wk.x * d1.y + wk.y * d1.x);
-            const float2 z1 = make_float2(s1.x - e1.y, s1.y + e1.x);  // s1 + i e1
-            if (k == 0) {
-                R[0] = z1;
-            } else {
-                const int k2 = M - k;  // Z[M-k] from (X[M-k], X[k]); w^(M-k) = conj(tw[M-k])
-                const float2 w2 = make_float2(tw[k2].x, -tw[k2].y);
-                const float2 s2 = make_float2(c.x + a.x, c.y - a.y);
-                const float2 d2 = make_float2(c.x - a.x, c.y + a.y);
-                const float2 e2 = make_float2(w2.x * d2.x - w2.y * d2.y, w2.x * d2.y + w2.y * d2.x);
-                R[k] = z1;
-                R[k2] = make_float2(s2.x - e2.y, s2.y + e2.x);
-            }
-        }
-        __syncthreads();
-        // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP)
+        // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP),
+        // its inputs formed by the complex-to-real pre-step as they are loaded:
+        // Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])), w^k = conj(tw[k])
         // zero-initialised so that no value of the arrays is carried around the plane loop
         float2 y[KZ][N2z] = {};
         {
@@ -474,7 +452,13 @@ __global__ void __launch_bounds__(1024)
                 if (u < N2z)
                     sfor<0, N1z>([&](auto jc) {
                         constexpr int j1 = decltype(jc)::value;
-                        v[j][j1] = P[line * LSP + u + N2z * j1];
+                        const int k = u + N2z * j1;
+                        const float2 a = P[line * LSP + k], c = P[line * LSP + M - k], w = tw[k];
+                        const float2 s1 = make_float2(a.x + c.x, a.y - c.y);  // X[k] + conj(X[M-k])
+                        const float2 d1 = make_float2(a.x - c.x, a.y + c.y);  // X[k] - conj(X[M-k])
+                        // e = conj(w) d1; Z = s1 + i e
+                        const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
+                        v[j][j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
                     });
             }
             __syncthreads();

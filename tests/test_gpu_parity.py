@@ -240,3 +240,41 @@ def test_dock_all_tied_scores(gpu, gcq, kernels, t, ng):
             assert g["score"] == 0.0, (q, k, g)
             assert list(g["index"]) == [int(ids[q]), 0, 0, k], (q, k, g)
     ctx.close()
+
+
+@pytest.mark.parametrize("nrot,t,n", [(360, 2, 720), (4096, 3, 10000), (4096, 3, 1)])
+def test_topn_device_merge(gpu, gcq, nrot, t, n):
+    """mdk_topn merges on the device: (score desc, rotation asc, rank asc), as a stable
+    sort of the per-rotation results by descending score.  n = 10000 takes the
+    global-memory sorting network (more than 8192 keys)."""
+    rec, lig, ng = gcq
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=64)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(54000)[:nrot]
+    ctx.set_rotations(M.rotation_coefs(ang))
+    ctx.dock_range(0, nrot)
+    ctx.sync()
+    res = ctx.results(0, nrot)
+    _say("topn %d of %d results" % (n, len(res)))
+    top = ctx.topn(nrot, n)
+    order = np.argsort(-res["score"].astype(np.float64), kind="stable")[:n]
+    assert np.array_equal(top.view(np.uint8), res[order].view(np.uint8))
+    ctx.close()
+
+
+def test_topn_device_merge_all_tied(gpu, gcq):
+    """Every score 0 (zero receptor spectrum): the merge keeps result order."""
+    rec, lig, ng = gcq
+    nf = 2 * ng
+    ctx = _lig_ctx(lig, ng, t=3, max_batch=16)
+    z = np.zeros(nf ** 3, np.float32)
+    ctx.set_receptor_spectrum(z, z)
+    ang = O.load_rotations(360)[:40]
+    ctx.set_rotations(M.rotation_coefs(ang))
+    ctx.dock_range(0, 40)
+    ctx.sync()
+    res = ctx.results(0, 40)
+    top = ctx.topn(40, 100)
+    assert np.array_equal(top.view(np.uint8), res[:100].view(np.uint8))
+    ctx.close()
