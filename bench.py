@@ -53,31 +53,20 @@ def stage_bytes(nf, ng, t):
 
 
 def rocfft_reference(nf, batch, reps=5):
-    """torch.fft (hipFFT/rocFFT) forward + inverse 3D C2C of `batch` nf^3 complex64 grids: the
-    vendor FFT alone, as a reference point next to the whole rotation loop (which also
-    voxelizes, multiplies, folds and reduces)."""
-    try:
-        import torch
-        if not torch.cuda.is_available():
-            return None
-        x = torch.randn(batch, nf, nf, nf, dtype=torch.complex64, device="cuda")
-        for _ in range(2):
-            y = torch.fft.ifftn(torch.fft.fftn(x, dim=(1, 2, 3)), dim=(1, 2, 3))
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            y = torch.fft.ifftn(torch.fft.fftn(x, dim=(1, 2, 3)), dim=(1, 2, 3))
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        del x, y
-        torch.cuda.empty_cache()
-        return {"what": "forward + inverse 3D C2C FFT of %d grids of %d^3 complex64 (torch.fft on rocFFT)" % (batch, nf),
-                "ms": round(ms, 3), "grids_per_s": round(batch / (ms * 1e-3), 1),
-                "GBps_at_B_rot": round(batch * 32.0 * nf ** 3 / (ms * 1e-3) / 1e9, 1)}
-    except Exception as e:  # the reference point is optional
-        return {"error": str(e)[:200]}
+    """The vendor FFT (hipFFT on rocFFT, megadock_amd/lib/libmdkfftref.so): forward + inverse
+    3D C2C of `batch` nf^3 complex64 grids, a reference point next to the whole rotation loop
+    (which also voxelizes, multiplies, folds and reduces)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "megadock_amd", "lib", "libmdkfftref.so")
+    if not os.path.exists(path):
+        return {"error": "libmdkfftref.so not built"}
+    ms = C.c_float(0.0)
+    rc = C.CDLL(path).mdkref_fft_time(0, nf, batch, reps, C.byref(ms))
+    if rc:
+        return {"error": "mdkref_fft_time returned %d" % rc}
+    return {"what": "forward + inverse 3D C2C FFT of %d grids of %d^3 complex64 (hipFFT/rocFFT)" % (batch, nf),
+            "ms": round(ms.value, 3), "grids_per_s": round(batch / (ms.value * 1e-3), 1),
+            "GBps_at_B_rot": round(batch * 32.0 * nf ** 3 / (ms.value * 1e-3) / 1e9, 1)}
 
 
 def main():
