@@ -81,9 +81,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the torch.fft (rocFFT) reference point")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
-    ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i) and "
-                    "batch sizes (b), e.g. 'v0,v3,i1,b64'; settings accumulate; prints one stage-timing line "
-                    "per setting and exits")
+    ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i), "
+                    "stream counts (s) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
+                    "one stage-timing line per setting and exits")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -127,6 +127,8 @@ def main():
                 os.environ["MDK_XCORR_VARIANT"] = item[1:]
             if item[0] == "i":
                 os.environ["MDK_INV_VARIANT"] = item[1:]
+            if item[0] == "s":
+                os.environ["MDK_STREAMS"] = item[1:]
             bs = int(item[1:]) if item[0] == "b" else B
             sctx = M.Context(nf, ng, lig.na, t=t, max_batch=bs)
             sctx.set_receptor(gr, gi)

@@ -383,10 +383,23 @@ bool make_plan(int n, FftPlan &p, std::vector<float2> &tw) {
 }  // namespace
 
 // ==================================================================== context
+// Buffers of one batch in flight, and the stream its kernels run on.
+struct BatchSlot {
+    hipStream_t stream = nullptr;
+    uint32_t *d_bits = nullptr;  // voxelization scratch, laid out by vs
+    VoxScratch vs{};
+    float *d_chg = nullptr;      // charge grids, max_batch * ng^3, zero between batches
+    int *d_vidx = nullptr;
+    float2 *d_lg = nullptr, *d_a1 = nullptr, *d_a2 = nullptr, *d_cc = nullptr;
+    Cand *d_cand = nullptr;
+    mdk_sortscore *d_bout = nullptr;  // max_batch * t
+    hipEvent_t done = nullptr;        // recorded after a range's last batch on this stream
+};
+
 struct mdk_ctx {
     int device = 0, nf = 0, ng = 0, na = 0, t = 1, max_batch = 1;
     int nd = 0, H = 0, Hp = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // slot[0].stream: set-up, single batches, copies
     FftPlan plan{};
     float2 *d_tw = nullptr;
     float2 *d_tw4 = nullptr;  // exp(-2 pi i m / nf): twiddles of the specialised pipeline
@@ -405,14 +418,10 @@ struct mdk_ctx {
     int *d_rotid = nullptr;  // iota table, rotation i has number i
     float *d_brot = nullptr; // staging for mdk_dock (max_batch * 9)
     int *d_bid = nullptr;    // staging ids
-    // batch buffers
-    uint32_t *d_bits = nullptr;  // voxelization scratch, laid out by vs
-    VoxScratch vs{};
-    float *d_chg = nullptr;      // charge grids, max_batch * ng^3, zero between batches
-    int *d_vidx = nullptr;
-    float2 *d_lg = nullptr, *d_a1 = nullptr, *d_a2 = nullptr, *d_cc = nullptr;
-    Cand *d_cand = nullptr;
-    mdk_sortscore *d_bout = nullptr;   // max_batch * t
+    // batch buffers: one set per stream; mdk_dock_range alternates batches between the
+    // streams so that one batch's memory-bound passes overlap another's compute-bound ones
+    BatchSlot slot[2];
+    int nslots = 1;                // MDK_STREAMS (1 or 2)
     mdk_sortscore *d_results = nullptr; // nrot * t
     void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
     mdk_sortscore *d_topn_out = nullptr;
@@ -426,17 +435,17 @@ size_t lds_bytes(const mdk_ctx *c, int lines) {
     return sizeof(float2) * ((size_t)c->plan.ntw + 2 * (size_t)c->nf * (lines + 1));
 }
 
-VoxGrid vox_grid(const mdk_ctx *c) {
+VoxGrid vox_grid(const mdk_ctx *c, const BatchSlot &S) {
     const int ng = c->ng;
-    return VoxGrid{c->vs.dmo, c->d_chg, ng, row_words(ng), mask_words(ng), ng * ng * ng, c->lig.delta};
+    return VoxGrid{S.vs.dmo, S.d_chg, ng, row_words(ng), mask_words(ng), ng * ng * ng, c->lig.delta};
 }
 
 // Run the forward ligand passes + correlation + inverse passes for B rotations that
 // launch_voxelize has voxelized; output top-t candidates (or the score map).
 // ev (optional): events recorded after each of the five FFT-side stages
-int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
+int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
-    hipStream_t s = c->stream;
+    hipStream_t s = S.stream;
     if (c->use_pipe) {
         PipeArgs a;
         a.B = B;
@@ -446,12 +455,12 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
         a.Hp = c->Hp;
         a.t = c->t;
         a.nf3f = (float)((size_t)nf * nf * nf);
-        a.vox = vox_grid(c);
-        a.a1 = c->d_a1;
-        a.a2 = c->d_a2;
-        a.cc = c->d_cc;
+        a.vox = vox_grid(c, S);
+        a.a1 = S.d_a1;
+        a.a2 = S.d_a2;
+        a.cc = S.d_cc;
         a.rec = c->d_rec;
-        a.cand = c->d_cand;
+        a.cand = S.d_cand;
         a.smap = smap_dev;
         a.tw4 = c->d_tw4;
         a.stream = s;
@@ -462,19 +471,19 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
         HIPCHK(pipe_run(nf, a));
         return 0;
     }
-    HIPCHK(launch_vox_expand(vox_grid(c), B, c->d_lg, s));
+    HIPCHK(launch_vox_expand(vox_grid(c, S), B, S.d_lg, s));
     const int lines_z = B * ng * ng;
-    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
+    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(S.d_lg, S.d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, B), NT, lds_bytes(c, WY), s>>>(c->d_a1, c->d_a2, c->plan, c->d_tw, ng, nd);
+    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, B), NT, lds_bytes(c, WY), s>>>(S.d_a1, S.d_a2, c->plan, c->d_tw, ng, nd);
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
-    k_x_corr<<<dim3(c->Hp / WX, nf, B), NT, lds_bytes(c, 2 * WX), s>>>(c->d_a2, c->d_rec, c->d_cc, c->plan, c->d_tw, ng, nd, c->H, c->Hp);
+    k_x_corr<<<dim3(c->Hp / WX, nf, B), NT, lds_bytes(c, 2 * WX), s>>>(S.d_a2, c->d_rec, S.d_cc, c->plan, c->d_tw, ng, nd, c->H, c->Hp);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    k_inv_y<<<dim3(c->Hp / WY, nf, B), NT, lds_bytes(c, WY), s>>>(c->d_cc, c->plan, c->d_tw, c->Hp);
+    k_inv_y<<<dim3(c->Hp / WY, nf, B), NT, lds_bytes(c, WY), s>>>(S.d_cc, c->plan, c->d_tw, c->Hp);
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
     const int rows = B * nf * nf;
     const size_t lds_z = lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * c->t;
-    k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(c->d_cc, c->d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, c->t,
+    k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(S.d_cc, S.d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, c->t,
                                                        (float)((size_t)nf * nf * nf));
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
     HIPCHK(hipGetLastError());
@@ -482,16 +491,16 @@ int run_fft_pipeline(mdk_ctx *c, int B, float *smap_dev, hipEvent_t *ev = nullpt
 }
 
 // ev (optional): MDK_NSTAGE + 1 events bracketing the stages of one batch
-int dock_batch(mdk_ctx *c, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev,
+int dock_batch(mdk_ctx *c, BatchSlot &S, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev,
                hipEvent_t *ev = nullptr) {
-    if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-    HIPCHK(launch_voxelize(c->lig, rot_dev, B, c->vs, c->d_vidx, c->d_chg, c->stream));
-    if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
-    if (run_fft_pipeline(c, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
+    if (ev) HIPCHK(hipEventRecord(ev[0], S.stream));
+    HIPCHK(launch_voxelize(c->lig, rot_dev, B, S.vs, S.d_vidx, S.d_chg, S.stream));
+    if (ev) HIPCHK(hipEventRecord(ev[1], S.stream));
+    if (run_fft_pipeline(c, S, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
     const int lists = c->use_pipe ? pipe_cands_per_rotation(nf) : nf * nf;
-    k_reduce_top<<<B, NT, 0, c->stream>>>(c->d_cand, out_dev, id_dev, lists, c->t, nf, (float)((size_t)nf * nf * nf));
-    if (ev) HIPCHK(hipEventRecord(ev[7], c->stream));
+    k_reduce_top<<<B, NT, 0, S.stream>>>(S.d_cand, out_dev, id_dev, lists, c->t, nf, (float)((size_t)nf * nf * nf));
+    if (ev) HIPCHK(hipEventRecord(ev[7], S.stream));
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -562,25 +571,33 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         return rc;
     };
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail("mdk_create: hipSetDevice failed"); }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return fail("mdk_create: stream creation failed");
+    {
+        const char *ns = std::getenv("MDK_STREAMS");
+        c->nslots = (ns && std::atoi(ns) == 2) ? 2 : 1;  // two streams measured no faster at 144^3
     }
+    for (int k = 0; k < c->nslots; k++)
+        if (hipStreamCreateWithFlags(&c->slot[k].stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->slot[k].done, hipEventDisableTiming) != hipSuccess)
+            return bail(fail("mdk_create: stream creation failed"));
+    c->stream = c->slot[0].stream;
     const size_t nf3 = (size_t)nf * nf * nf, ng3 = (size_t)ng * ng * ng, B = max_batch;
     if (dmalloc(&c->d_tw, tw.size()) || dmalloc(&c->d_rec, nf3) || dmalloc(&c->d_xyz, 3 * (size_t)na) ||
         dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
-        dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) ||
-        dmalloc(&c->d_bits, B * vox_scratch_words(ng)) || dmalloc(&c->d_vidx, B * na) ||
-        dmalloc(&c->d_lg, B * ng3) || dmalloc(&c->d_chg, B * ng3) || dmalloc(&c->d_a1, B * ng * ng * nf) ||
-        dmalloc(&c->d_a2, B * ng * nf * nf) || dmalloc(&c->d_cc, B * nf * nf * c->Hp) ||
-        dmalloc(&c->d_cand, B * nf * nf * t) || dmalloc(&c->d_bout, B * t))
+        dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B))
         return bail(1);
-    c->vs = vox_scratch(c->d_bits, B, ng);
-    // the atom counts and the charge grids start, and are kept between batches, all-zero
-    if (hipMemsetAsync(c->d_bits, 0, sizeof(uint32_t) * B * vox_scratch_words(ng), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->d_chg, 0, sizeof(float) * B * ng3, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-        return bail(fail("mdk_create: scratch initialisation failed"));
+    for (int k = 0; k < c->nslots; k++) {
+        BatchSlot &S = c->slot[k];
+        if (dmalloc(&S.d_bits, B * vox_scratch_words(ng)) || dmalloc(&S.d_vidx, B * na) || dmalloc(&S.d_lg, B * ng3) ||
+            dmalloc(&S.d_chg, B * ng3) || dmalloc(&S.d_a1, B * ng * ng * nf) || dmalloc(&S.d_a2, B * ng * nf * nf) ||
+            dmalloc(&S.d_cc, B * nf * nf * c->Hp) || dmalloc(&S.d_cand, B * nf * nf * t) || dmalloc(&S.d_bout, B * t))
+            return bail(1);
+        S.vs = vox_scratch(S.d_bits, B, ng);
+        // the atom counts and the charge grids start, and are kept between batches, all-zero
+        if (hipMemsetAsync(S.d_bits, 0, sizeof(uint32_t) * B * vox_scratch_words(ng), S.stream) != hipSuccess ||
+            hipMemsetAsync(S.d_chg, 0, sizeof(float) * B * ng3, S.stream) != hipSuccess ||
+            hipStreamSynchronize(S.stream) != hipSuccess)
+            return bail(fail("mdk_create: scratch initialisation failed"));
+    }
     if (hipMemcpy(c->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail("mdk_create: twiddle upload failed"));
     {
@@ -619,15 +636,22 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
 int mdk_destroy(mdk_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; k++)
+        if (c->slot[k].stream) (void)hipStreamSynchronize(c->slot[k].stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
-                    c->d_rotid, c->d_brot, c->d_bid, c->d_bits, c->d_chg, c->d_vidx, c->d_lg, c->d_a1, c->d_a2,
-                    c->d_cc, c->d_cand, c->d_bout, c->d_results, c->d_topn_scratch, c->d_topn_out};
+                    c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (int k = 0; k < 2; k++) {
+        BatchSlot &S = c->slot[k];
+        void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout};
+        for (void *p : sp)
+            if (p) (void)hipFree(p);
+        if (S.done) (void)hipEventDestroy(S.done);
+        if (S.stream) (void)hipStreamDestroy(S.stream);
+    }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -639,11 +663,11 @@ int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     const size_t ng3 = (size_t)ng * ng * ng;
     std::vector<float2> h(ng3);
     for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
-    HIPCHK(hipMemcpyAsync(c->d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->slot[0].d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
     const int lines_z = ng * ng;
-    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), c->stream>>>(c->d_lg, c->d_a1, c->plan, c->d_tw, ng, nd, lines_z);
-    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->d_a1, c->d_a2, c->plan, c->d_tw, ng, nd);
-    k_fwd_x_plain<<<dim3((nf + WY - 1) / WY, nf), NT, lds_bytes(c, WY), c->stream>>>(c->d_a2, c->d_rec, c->plan, c->d_tw, ng, nd);
+    k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), c->stream>>>(c->slot[0].d_lg, c->slot[0].d_a1, c->plan, c->d_tw, ng, nd, lines_z);
+    k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a1, c->slot[0].d_a2, c->plan, c->d_tw, ng, nd);
+    k_fwd_x_plain<<<dim3((nf + WY - 1) / WY, nf), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a2, c->d_rec, c->plan, c->d_tw, ng, nd);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_rec = true;
@@ -717,8 +741,8 @@ int mdk_dock(mdk_ctx *c, const float *rotcoef, const int *rot_id, int B, mdk_sor
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_bid, rot_id, sizeof(int) * B, hipMemcpyHostToDevice, c->stream));
-    if (dock_batch(c, c->d_brot, c->d_bid, B, c->d_bout)) return 1;
-    HIPCHK(hipMemcpyAsync(out, c->d_bout, sizeof(mdk_sortscore) * B * c->t, hipMemcpyDeviceToHost, c->stream));
+    if (dock_batch(c, c->slot[0], c->d_brot, c->d_bid, B, c->slot[0].d_bout)) return 1;
+    HIPCHK(hipMemcpyAsync(out, c->slot[0].d_bout, sizeof(mdk_sortscore) * B * c->t, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -746,16 +770,27 @@ int mdk_dock_range(mdk_ctx *c, int first, int count) {
     if (!c->have_rec || !c->have_lig || !c->d_rot) return fail("mdk_dock_range: receptor, ligand and rotations must be set first");
     if (first < 0 || count < 0 || first + count > c->nrot) return fail("mdk_dock_range: range outside the rotation table");
     HIPCHK(hipSetDevice(c->device));
-    for (int b0 = first; b0 < first + count; b0 += c->max_batch) {
+    // the other streams start after everything queued on the main stream so far
+    HIPCHK(hipEventRecord(c->slot[0].done, c->stream));
+    for (int k = 1; k < c->nslots; k++) HIPCHK(hipStreamWaitEvent(c->slot[k].stream, c->slot[0].done, 0));
+    int q = 0;
+    for (int b0 = first; b0 < first + count; b0 += c->max_batch, q++) {
         const int B = std::min(c->max_batch, first + count - b0);
-        if (dock_batch(c, c->d_rot + 9 * (size_t)b0, c->d_rotid + b0, B, c->d_results + (size_t)b0 * c->t)) return 1;
+        if (dock_batch(c, c->slot[q % c->nslots], c->d_rot + 9 * (size_t)b0, c->d_rotid + b0, B,
+                       c->d_results + (size_t)b0 * c->t))
+            return 1;
+    }
+    // and the main stream continues after all of them (results, merge, timing events)
+    for (int k = 1; k < c->nslots; k++) {
+        HIPCHK(hipEventRecord(c->slot[k].done, c->slot[k].stream));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->slot[k].done, 0));
     }
     return 0;
 }
 
 int mdk_sync(mdk_ctx *c) {
     if (!c) return fail("mdk_sync: null context");
-    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->nslots; k++) HIPCHK(hipStreamSynchronize(c->slot[k].stream));
     return 0;
 }
 
@@ -815,7 +850,7 @@ int mdk_profile_stages(mdk_ctx *c, int first, int count, float *ms, int *nstages
     int made = 0, rc = 0;
     for (; made < MDK_NSTAGE + 1; made++)
         if (hipEventCreate(&ev[made]) != hipSuccess) { rc = fail("mdk_profile_stages: event creation failed"); break; }
-    if (!rc) rc = dock_batch(c, c->d_rot + 9 * (size_t)first, c->d_rotid + first, count, c->d_results + (size_t)first * c->t, ev);
+    if (!rc) rc = dock_batch(c, c->slot[0], c->d_rot + 9 * (size_t)first, c->d_rotid + first, count, c->d_results + (size_t)first * c->t, ev);
     if (!rc && hipEventSynchronize(ev[MDK_NSTAGE]) != hipSuccess) rc = fail("mdk_profile_stages: sync failed");
     for (int q = 0; !rc && q < MDK_NSTAGE; q++)
         if (hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]) != hipSuccess) rc = fail("mdk_profile_stages: timing failed");
@@ -830,11 +865,11 @@ int mdk_voxelize(mdk_ctx *c, const float *rotcoef, int B, float *grid_r, float *
     if (B < 1 || B > c->max_batch) return fail("mdk_voxelize: batch size out of range");
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, c->vs, c->d_vidx, c->d_chg, c->stream));
-    HIPCHK(launch_vox_expand(vox_grid(c), B, c->d_lg, c->stream));
+    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, c->slot[0].vs, c->slot[0].d_vidx, c->slot[0].d_chg, c->stream));
+    HIPCHK(launch_vox_expand(vox_grid(c, c->slot[0]), B, c->slot[0].d_lg, c->stream));
     const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
     std::vector<float2> h(B * ng3);
-    HIPCHK(hipMemcpyAsync(h.data(), c->d_lg, sizeof(float2) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h.data(), c->slot[0].d_lg, sizeof(float2) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < h.size(); i++) {
         grid_r[i] = h[i].x;
@@ -852,9 +887,9 @@ int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
     HIPCHK(hipMalloc(&d_map, sizeof(float) * nf3));
     int rc = 0;
     if (hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        launch_voxelize(c->lig, c->d_brot, 1, c->vs, c->d_vidx, c->d_chg, c->stream) != hipSuccess)
+        launch_voxelize(c->lig, c->d_brot, 1, c->slot[0].vs, c->slot[0].d_vidx, c->slot[0].d_chg, c->stream) != hipSuccess)
         rc = fail("mdk_score_map: voxelization failed");
-    if (!rc) rc = run_fft_pipeline(c, 1, d_map);
+    if (!rc) rc = run_fft_pipeline(c, c->slot[0], 1, d_map);
     if (!rc && (hipMemcpyAsync(out, d_map, sizeof(float) * nf3, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipStreamSynchronize(c->stream) != hipSuccess))
         rc = fail("mdk_score_map: copy back failed");

@@ -126,17 +126,21 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     const int c = B_FAST ? blockIdx.y : blockIdx.x, ky = B_FAST ? blockIdx.z : blockIdx.y;
     const int b0 = (B_FAST ? blockIdx.x : blockIdx.z) * RB;
     const int kyp = (N - ky) % N;
+    // fold elements of this thread: column l (fixed), rows kx = fx0 + FX p, p < PER
+    static_assert(NT % L == 0, "fold mapping");
+    constexpr int FX = NT / L;
+    const int fl = tid % L, fx0 = tid / L, fkz = c * L + fl;
+    const bool fcol = fkz < H;
+    const unsigned fa0 = 8u * (unsigned)((fx0 * N + ky) * N + fkz);                    // + 8 FX N^2 p
+    const unsigned fd0 = 8u * (unsigned)(((N - fx0) * N + kyp) * N + (N - fkz) % N);  // - 8 FX N^2 p
     // receptor values of this thread's fold elements (kept in registers when CACHE_R)
     auto rec_pair = [&](int p, float2 &a, float2 &d) {
-        const int q = tid + p * NT;
         a = d = make_float2(0.f, 0.f);
-        if (q < L * N) {
-            const int l = q % L, kx = q / L, kz = c * L + l;
-            if (kz < H) {
-                const int kxp = (N - kx) % N, kzp = (N - kz) % N;
-                a = ld_at(rec, 8u * (unsigned)((kx * N + ky) * N + kz));
-                d = ld_at(rec, 8u * (unsigned)((kxp * N + kyp) * N + kzp));
-            }
+        if (fx0 + FX * p < N && fcol) {
+            a = ld_at(rec, fa0 + 8u * (unsigned)(FX * p * N * N));
+            // partner row -kx: N - kx for kx > 0 and 0 for kx = 0
+            const unsigned od = (fx0 + FX * p == 0) ? fd0 - 8u * (unsigned)(N * N * N) : fd0 - 8u * (unsigned)(FX * p * N * N);
+            d = ld_at(rec, od);
         }
     };
     float2 r1[CACHE_R ? PER : 1], r2[CACHE_R ? PER : 1];
@@ -179,10 +183,10 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
         float2 qv[PER];
 #pragma unroll
         for (int p = 0; p < PER; p++) {
-            const int q = tid + p * NT;
-            if (q < L * N) {
-                const int l = q % L, kx = q / L, kxp = (N - kx) % N;
-                const float2 l1 = S[kx * LS2 + l], l2 = S[kxp * LS2 + L + l];
+            const int kx = fx0 + FX * p;
+            if (kx < N) {
+                const int kxp = kx == 0 ? 0 : N - kx;
+                const float2 l1 = S[kx * LS2 + fl], l2 = S[kxp * LS2 + L + fl];
                 float2 a, d;
                 if constexpr (CACHE_R) {
                     a = r1[p];
@@ -197,10 +201,8 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
         }
         if (QA) __syncthreads();  // every read of S above is done before Q overwrites it
 #pragma unroll
-        for (int p = 0; p < PER; p++) {
-            const int q = tid + p * NT;
-            if (q < L * N) Q[(q / L) * LS + q % L] = qv[p];
-        }
+        for (int p = 0; p < PER; p++)
+            if (fx0 + FX * p < N) Q[(fx0 + FX * p) * LS + fl] = qv[p];
         __syncthreads();
         // inverse x-FFT of the L lines of Q; thread (lq, uq) over the first L*T threads
         const int lq = tid % L, uq = tid / L;
