@@ -112,6 +112,23 @@ int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float
  * rotation, i.e. Re IFFT(conj(R) * FFT(ligand)). */
 int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
 
+/* Off-grid score of B poses with its translation gradient, for refining decoys.  No
+ * reference counterpart (extends the score of src/fft_process.cpp:333-446).  Pose b is
+ * rotation rotcoef[9b..9b+8] (row-major, as mdk_dock) and a continuous translation
+ * t[3b..3b+2] in voxels, the units of a result's index[1..3].  out[4b..4b+3] =
+ * (s, ds/dtx, ds/dty, ds/dtz) where s is the band-limited interpolation of the hard-voxel
+ * score map (value / nf^3, equal to the reported score at grid points).  Needs nf = 80,
+ * 96, 144 or 256. */
+int mdk_score_spectral(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
+
+/* Companion score of B poses (same pose arguments) with the ligand atoms spread
+ * trilinearly instead of voxelized: s = sum_a [w_a Rr(P_a) + q_a Ri(P_a)] with Rr, Ri the
+ * receptor grids of mdk_set_receptor interpolated at the placed atom P_a = R(a - c) -
+ * gw t' (t' the signed translation), w_a = (4/3) pi r_a^3 / gw^3.  out[7b..7b+6] =
+ * (s, ds/dt' (3, per voxel), ds/dw (3, per radian of a rotation exp([w]x) applied after
+ * R)).  No reference counterpart: the differentiable companion used to refine poses. */
+int mdk_score_companion(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
+
 /* Measurement hook: device-to-device copy of `bytes` (read + write), `reps` times;
  * *ms = mean milliseconds per copy.  Calibrates HBM counters and the achievable copy
  * bandwidth.  No reference counterpart. */

@@ -21,7 +21,7 @@ ABI_SYMBOLS = [
     "mdk_set_receptor", "mdk_set_receptor_spectrum", "mdk_get_receptor_spectrum",
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
-    "mdk_score_map", "mdk_fft1d", "mdk_copy_probe",
+    "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -177,6 +177,25 @@ class Context:
         gi = np.zeros(B * self.ng ** 3, np.float32)
         _check(lib().mdk_voxelize(self.h, _fp(rc), C.c_int(B), _fp(gr), _fp(gi)), "mdk_voxelize")
         return gr.reshape(B, -1), gi.reshape(B, -1)
+
+    def score_spectral(self, rotcoef, t):
+        """(B, 4): band-limited score at continuous translations t (B, 3) in voxels and
+        its gradient (s, ds/dtx, ds/dty, ds/dtz)."""
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        tt = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        assert len(rc) == len(tt)
+        out = np.zeros((len(rc), 4), np.float32)
+        _check(lib().mdk_score_spectral(self.h, _fp(rc), _fp(tt), C.c_int(len(rc)), _fp(out)), "mdk_score_spectral")
+        return out
+
+    def score_companion(self, rotcoef, t):
+        """(B, 7): trilinear companion score and its gradient (s, ds/dt' (3), ds/dw (3))."""
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        tt = np.ascontiguousarray(t, np.float32).reshape(-1, 3)
+        assert len(rc) == len(tt)
+        out = np.zeros((len(rc), 7), np.float32)
+        _check(lib().mdk_score_companion(self.h, _fp(rc), _fp(tt), C.c_int(len(rc)), _fp(out)), "mdk_score_companion")
+        return out
 
     def score_map(self, rotcoef):
         rc = np.ascontiguousarray(rotcoef, np.float32).reshape(9)

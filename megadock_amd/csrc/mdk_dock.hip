@@ -33,6 +33,13 @@ size_t topn_scratch_bytes(int n);
 hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *scratch, mdk_sortscore *out,
                        hipStream_t s);
 
+// off-grid scores (mdk_grad.hip)
+size_t grad_scratch_doubles(int max_batch);
+hipError_t launch_spec_eval(const float2 *Q, const float *t_dev, int B, int nf, int H, int Hp, double *part,
+                            float *out_dev, hipStream_t s);
+hipError_t launch_companion(const LigandDev &lig, const float *wshape, const float2 *recgrid, const float *rot_dev,
+                            const float *t_dev, int B, int nf, double *part, float *out_dev, hipStream_t s);
+
 #define MDK_NSTAGE 7  // voxelize, fwd_z, fwd_y, x_corr, inv_y, inv_z_top, reduce
 
 namespace {
@@ -423,6 +430,13 @@ struct mdk_ctx {
     BatchSlot slot[2];
     int nslots = 1;                // MDK_STREAMS (1 or 2)
     mdk_sortscore *d_results = nullptr; // nrot * t
+    // off-grid scores and gradients (mdk_grad.hip)
+    float2 *d_recgrid = nullptr;  // receptor grids (grid_r, grid_i), ng^3, real space
+    float *d_wshape = nullptr;    // companion shape weight per ligand atom
+    float *d_bt = nullptr;        // translations of a batch (max_batch * 3)
+    float *d_gout = nullptr;      // results of a batch (max_batch * 7)
+    double *d_gpart = nullptr;    // partial sums
+    bool have_recgrid = false;
     void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
     mdk_sortscore *d_topn_out = nullptr;
     int topn_cap = 0;
@@ -443,7 +457,9 @@ VoxGrid vox_grid(const mdk_ctx *c, const BatchSlot &S) {
 // Run the forward ligand passes + correlation + inverse passes for B rotations that
 // launch_voxelize has voxelized; output top-t candidates (or the score map).
 // ev (optional): events recorded after each of the five FFT-side stages
-int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_t *ev = nullptr) {
+// spectrum_only: stop after the fold, leaving Q[b][kx][ky][kz] in S.d_cc (specialised sizes)
+int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_t *ev = nullptr,
+                     bool spectrum_only = false) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     hipStream_t s = S.stream;
     if (c->use_pipe) {
@@ -468,6 +484,7 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
         a.variant = c->xcorr_variant;
         a.cu_count = c->cu_count;
         a.inv_variant = c->inv_variant;
+        a.spectrum_only = spectrum_only;
         HIPCHK(pipe_run(nf, a));
         return 0;
     }
@@ -583,7 +600,9 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     const size_t nf3 = (size_t)nf * nf * nf, ng3 = (size_t)ng * ng * ng, B = max_batch;
     if (dmalloc(&c->d_tw, tw.size()) || dmalloc(&c->d_rec, nf3) || dmalloc(&c->d_xyz, 3 * (size_t)na) ||
         dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
-        dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B))
+        dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) || dmalloc(&c->d_recgrid, ng3) ||
+        dmalloc(&c->d_wshape, na) || dmalloc(&c->d_bt, 3 * B) || dmalloc(&c->d_gout, 7 * B) ||
+        dmalloc(&c->d_gpart, grad_scratch_doubles((int)B)))
         return bail(1);
     for (int k = 0; k < c->nslots; k++) {
         BatchSlot &S = c->slot[k];
@@ -639,7 +658,8 @@ int mdk_destroy(mdk_ctx *c) {
     for (int k = 0; k < 2; k++)
         if (c->slot[k].stream) (void)hipStreamSynchronize(c->slot[k].stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
-                    c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out};
+                    c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out,
+                    c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 2; k++) {
@@ -664,6 +684,7 @@ int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     std::vector<float2> h(ng3);
     for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
     HIPCHK(hipMemcpyAsync(c->slot[0].d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_recgrid, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
     const int lines_z = ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), c->stream>>>(c->slot[0].d_lg, c->slot[0].d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a1, c->slot[0].d_a2, c->plan, c->d_tw, ng, nd);
@@ -671,6 +692,7 @@ int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_rec = true;
+    c->have_recgrid = true;
     return 0;
 }
 
@@ -682,6 +704,7 @@ int mdk_set_receptor_spectrum(mdk_ctx *c, const float *re, const float *im) {
     for (size_t i = 0; i < nf3; i++) h[i] = make_float2(re[i], im[i]);
     HIPCHK(hipMemcpy(c->d_rec, h.data(), sizeof(float2) * nf3, hipMemcpyHostToDevice));
     c->have_rec = true;
+    c->have_recgrid = false;  // no real-space grids behind this spectrum
     return 0;
 }
 
@@ -716,6 +739,12 @@ int mdk_set_ligand(mdk_ctx *c, const float *xyz, const float *center, const floa
     HIPCHK(hipMemcpy(c->d_rs2, rs2.data(), sizeof(float) * na, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_charge, charge, sizeof(float) * na, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_gc, grid_coord, sizeof(float) * c->ng, hipMemcpyHostToDevice));
+    {
+        std::vector<float> ws(na);  // companion shape weight: the atom's volume in voxels
+        for (int i = 0; i < na; i++)
+            ws[i] = (float)(4.0 / 3.0 * M_PI * (double)radius[i] * radius[i] * radius[i] / ((double)gw * gw * gw));
+        HIPCHK(hipMemcpy(c->d_wshape, ws.data(), sizeof(float) * na, hipMemcpyHostToDevice));
+    }
     LigandDev &L = c->lig;
     L.na = na;
     L.ng = c->ng;
@@ -895,6 +924,38 @@ int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
         rc = fail("mdk_score_map: copy back failed");
     (void)hipFree(d_map);
     return rc;
+}
+
+int mdk_score_spectral(mdk_ctx *c, const float *rotcoef, const float *t, int B, float *out) {
+    if (!c || !rotcoef || !t || !out) return fail("mdk_score_spectral: null argument");
+    if (!c->have_rec || !c->have_lig) return fail("mdk_score_spectral: receptor and ligand must be set first");
+    if (B < 1 || B > c->max_batch) return fail("mdk_score_spectral: batch size out of range");
+    if (!c->use_pipe) return fail("mdk_score_spectral: needs the specialised kernels (nf = 80, 96, 144 or 256)");
+    HIPCHK(hipSetDevice(c->device));
+    BatchSlot &S = c->slot[0];
+    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bt, t, sizeof(float) * 3 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, S.vs, S.d_vidx, S.d_chg, c->stream));
+    if (run_fft_pipeline(c, S, B, nullptr, nullptr, true)) return 1;
+    HIPCHK(launch_spec_eval(S.d_cc, c->d_bt, B, c->nf, c->H, c->Hp, c->d_gpart, c->d_gout, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->d_gout, sizeof(float) * 4 * B, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_score_companion(mdk_ctx *c, const float *rotcoef, const float *t, int B, float *out) {
+    if (!c || !rotcoef || !t || !out) return fail("mdk_score_companion: null argument");
+    if (!c->have_recgrid || !c->have_lig)
+        return fail("mdk_score_companion: receptor grids (mdk_set_receptor) and ligand must be set first");
+    if (B < 1 || B > c->max_batch) return fail("mdk_score_companion: batch size out of range");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bt, t, sizeof(float) * 3 * B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_companion(c->lig, c->d_wshape, c->d_recgrid, c->d_brot, c->d_bt, B, c->nf, c->d_gpart, c->d_gout,
+                            c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->d_gout, sizeof(float) * 7 * B, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int mdk_copy_probe(int device, size_t bytes, int reps, float *ms) {

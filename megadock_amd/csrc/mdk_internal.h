@@ -260,6 +260,7 @@ struct PipeArgs {
     int variant;        // kernel variant of the correlation pass (MDK_XCORR_VARIANT)
     int cu_count;       // compute units of the device (persistent grids)
     int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches (MDK_INV_VARIANT)
+    bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.

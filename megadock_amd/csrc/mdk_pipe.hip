@@ -109,7 +109,9 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
 // B_FAST: the rotation is the fastest grid index, so the workgroups that share a
 // receptor tile run back to back and find it in L2 / the last-level cache.
 // QA: the folded lines Q share the LDS of S (two more barriers, a third less LDS).
-template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false>
+// SPEC: store the folded spectrum Q itself, cc[b][kx][ky][kz], and skip the inverse x-FFT
+// (the spectral score evaluation reads it).
+template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false, bool SPEC = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
               const float2 *__restrict__ twg, int H, int Hp, int B) {
@@ -198,6 +200,14 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
                 const float2 p2 = make_float2(d.x * l2.x + d.y * l2.y, d.x * l2.y - d.y * l2.x);
                 qv[p] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
             }
+        }
+        if constexpr (SPEC) {
+            float2 *dst = cc + (size_t)b * N * N * Hp;
+            if (fkz < Hp)
+#pragma unroll
+                for (int p = 0; p < PER; p++)
+                    if (fx0 + FX * p < N) st_at(dst, 8u * (unsigned)(((fx0 + FX * p) * N + ky) * Hp + fkz), qv[p]);
+            continue;
         }
         if (QA) __syncthreads();  // every read of S above is done before Q overwrites it
 #pragma unroll
@@ -545,6 +555,11 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[0], s);
     kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
+    if (a.spectrum_only) {
+        kp_x_corr<N, LX, 1, false, true, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
+                                                                                                  a.H, a.Hp, B);
+        return hipGetLastError();
+    }
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
     // the grid, so consecutive workgroups share the receptor tile in L2, folded lines kept
     // in the exchange buffer's LDS (4, the default); the same with a separate fold buffer

@@ -357,3 +357,100 @@ def autogrid(rec, lig, gw=1.2, table=GRID_TABLE_11):
                 return v
         return n
     return max(pick(ngr), pick(ngl))
+
+
+# ---------------------------------------------------------------- off-grid scores (FP64)
+# The reference has no off-grid score; these FP64 restatements check the device kernels
+# for scores between grid points and their gradients.  They are pinned by the hard score
+# map at grid points, by central finite differences and by closed forms (tests).
+
+def signed_freq(nf):
+    k = np.arange(nf)
+    return np.where(k < nf // 2, k, k - nf)
+
+
+def folded_spectrum_fp64(nf, rec_grid_c, lig_grid_c):
+    """Q = (P(k) + conj(P(-k))) / 2 with P = conj(FFT r) FFT l, FP64, full nf^3."""
+    R = np.fft.fftn(np.asarray(rec_grid_c, np.complex128).reshape(nf, nf, nf))
+    L = np.fft.fftn(np.asarray(lig_grid_c, np.complex128).reshape(nf, nf, nf))
+    P = np.conj(R) * L
+    Pm = np.roll(np.flip(P, axis=(0, 1, 2)), 1, axis=(0, 1, 2))  # P(-k)
+    return 0.5 * (P + np.conj(Pm))
+
+
+def spectral_score_fp64(nf, Q, t):
+    """Band-limited score s(t) = 1/nf^3 sum_{kz <= nf/2} w(kz) Re[Q e^{i phi}] at a
+    continuous translation t (voxels) and its gradient, phi = 2 pi k'.t / nf with signed
+    frequencies k', w = 1 at kz = 0 and nf/2, else 2."""
+    H = nf // 2 + 1
+    k = signed_freq(nf).astype(np.float64)
+    ex = np.exp(2j * np.pi * k * t[0] / nf)
+    ey = np.exp(2j * np.pi * k * t[1] / nf)
+    ez = np.exp(2j * np.pi * k[:H] * t[2] / nf)
+    w = np.full(H, 2.0)
+    w[0] = 1.0
+    w[nf // 2] = 1.0
+    z = Q[:, :, :H] * ex[:, None, None] * ey[None, :, None] * (ez * w)[None, None, :]
+    n3 = float(nf) ** 3
+    s = np.real(z).sum() / n3
+    im = np.imag(z)
+    g = -2.0 * np.pi / nf / n3
+    grad = np.array([(im.sum(axis=(1, 2)) * k).sum(), (im.sum(axis=(0, 2)) * k).sum(),
+                     (im.sum(axis=(0, 1)) * k[:H]).sum()]) * g
+    return s, grad
+
+
+def trilinear_fp64(grid, ng, u):
+    """Value and gradient (per voxel) of the trilinear interpolation of an ng^3 grid at
+    voxel coordinate u (3,); corners outside the grid count as 0."""
+    i0 = np.floor(u).astype(np.int64)
+    f = u - i0
+    val = 0.0
+    grad = np.zeros(3)
+    g3 = grid.reshape(ng, ng, ng)
+    for c in range(8):
+        d = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+        i = i0 + d
+        if np.any(i < 0) or np.any(i >= ng):
+            continue
+        g = float(g3[i[0], i[1], i[2]])
+        wv = np.where(d == 1, f, 1.0 - f)
+        sv = np.where(d == 1, 1.0, -1.0)
+        val += wv.prod() * g
+        grad += np.array([sv[0] * wv[1] * wv[2], wv[0] * sv[1] * wv[2], wv[0] * wv[1] * sv[2]]) * g
+    return val, grad
+
+
+def companion_fp64(gr, gi, ng, gw, xyz, center_, radius, charge, R, t, nf):
+    """Companion score s = sum_a [w_a Rr(P_a) + q_a Ri(P_a)], P_a = R (a - c) - gw t',
+    w_a = (4/3) pi r_a^3 / gw^3, with (s, ds/dt' (3), ds/dw (3)) for a rotation exp([w]x)
+    applied after R."""
+    R = np.asarray(R, np.float64).reshape(3, 3)
+    t = np.asarray(t, np.float64)
+    tp = np.where(t < 0.5 * nf, t, t - nf)
+    X = (np.asarray(xyz, np.float64) - np.asarray(center_, np.float64)) @ R.T
+    P = X - gw * tp
+    u = P / gw + 0.5 * ng - 0.5
+    ws = 4.0 / 3.0 * np.pi * np.asarray(radius, np.float64) ** 3 / gw ** 3
+    q = np.asarray(charge, np.float64)
+    s = 0.0
+    dt = np.zeros(3)
+    dw = np.zeros(3)
+    for a in range(len(X)):
+        vr, gr_ = trilinear_fp64(gr, ng, u[a])
+        vi, gi_ = trilinear_fp64(gi, ng, u[a])
+        s += ws[a] * vr + q[a] * vi
+        g = (ws[a] * gr_ + q[a] * gi_) / gw
+        dt += -gw * g
+        dw += np.cross(X[a], g)
+    return s, dt, dw
+
+
+def rotvec_matrix(w):
+    """exp([w]x) (Rodrigues)."""
+    w = np.asarray(w, np.float64)
+    th = np.linalg.norm(w)
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if th < 1e-12:
+        return np.eye(3) + K
+    return np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * (K @ K)
