@@ -2,6 +2,8 @@
 
   c0  plain, small: 200 / 80 residues, auto grid (about 96^3), 3600 rotations, t=1, N=2000
   c1  plain, medium: 600 / 250 residues, -F 144 (144^3), 54000 rotations, t=3, N=2000
+  c2  all-to-all: 32 receptors x 32 ligands, residue counts log-uniform over [50, 800]
+      (drawn with seed 2999), auto grids, 3600 rotations per pair, t=1
   c4  large: 2000 / 1000 residues, -F 256 (256^3), 54000 rotations, t=3, N=2000
 
 Proteins come from megadock_amd.synth with seed = 1000*config + 10*index + role
@@ -52,3 +54,29 @@ def make_pair(name, outdir):
         with open(lig, "w") as f:
             f.write(synth.pdb_text(atoms))
     return rec, lig, cfg
+
+
+C2_PROTEINS = 32
+
+
+def c2_sizes():
+    """Residue counts of the c2 receptors (first 32) and ligands (next 32)."""
+    rng = np.random.default_rng(2999)
+    n = np.exp(rng.uniform(np.log(50.0), np.log(800.0), 2 * C2_PROTEINS))
+    return [int(round(x)) for x in n[:C2_PROTEINS]], [int(round(x)) for x in n[C2_PROTEINS:]]
+
+
+def make_c2_pair(receptor, ligand, outdir):
+    """Write the c2 receptor `receptor` and ligand `ligand` (0..31) as PDB files (if absent);
+    return their paths.  Settings: auto grid, 3600 rotations, t=1."""
+    os.makedirs(outdir, exist_ok=True)
+    rs, ls = c2_sizes()
+    rec = os.path.join(outdir, "c2_r%02d.pdb" % receptor)
+    lig = os.path.join(outdir, "c2_l%02d.pdb" % ligand)
+    if not os.path.exists(rec):
+        with open(rec, "w") as f:
+            f.write(synth.pdb_text(synth.build_protein(rs[receptor], synth.workload_seed(2, receptor, 0))))
+    if not os.path.exists(lig):
+        with open(lig, "w") as f:
+            f.write(synth.pdb_text(synth.build_protein(ls[ligand], synth.workload_seed(2, ligand, 1))))
+    return rec, lig

@@ -278,3 +278,39 @@ def test_topn_device_merge_all_tied(gpu, gcq):
     top = ctx.topn(40, 100)
     assert np.array_equal(top.view(np.uint8), res[:100].view(np.uint8))
     ctx.close()
+
+
+@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "0123"] +
+                         [{"MDK_INV_VARIANT": "1"}, {"MDK_STREAMS": "2"}])
+def test_tuning_variants_give_identical_results(gpu, gcq, env):
+    """Every kernel variant computes the same operations in the same order: results are
+    bitwise equal to the default's."""
+    rec, lig, _ = gcq
+    ng = 48
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ang = O.load_rotations(3600)[:300]
+    rc = M.rotation_coefs(ang)
+
+    def run():
+        ctx = _lig_ctx(lig, ng, t=3, max_batch=64)
+        ctx.set_receptor(gr, gi)
+        ctx.set_rotations(rc)
+        ctx.dock_range(0, len(rc))
+        ctx.sync()
+        out = ctx.results(0, len(rc))
+        ctx.close()
+        return out
+
+    base = run()
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        _say("variant %s" % env)
+        other = run()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    assert np.array_equal(base.view(np.uint8), other.view(np.uint8))
