@@ -79,10 +79,11 @@ def main():
     ap.add_argument("--plain-rotations", type=int, default=200)
     ap.add_argument("--cpu-rotations", type=int, default=0, help="CPU baseline sample size (0: automatic)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-rocfft", action="store_true", help="skip the torch.fft (rocFFT) reference point")
+    ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT (hipFFT) reference point")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
     ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i), "
-                    "stream counts (s) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
+                    "stream counts (s), graph replay on/off (g) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
                     "one stage-timing line per setting and exits")
     args = ap.parse_args()
 
@@ -116,6 +117,10 @@ def main():
     rot = M.rotation_coefs(angles)
     ctx.set_rotations(rot)
     log("context ready: batch %d" % B)
+    tuned = None
+    if not args.no_autotune and not args.sweep:
+        tuned = ctx.autotune(0, min(R, 16 * B))
+        log("autotune: correlation variant %d, inverse variant %d" % tuned)
 
     if args.sweep:
         nbytes = 1 << 30
@@ -129,6 +134,8 @@ def main():
                 os.environ["MDK_INV_VARIANT"] = item[1:]
             if item[0] == "s":
                 os.environ["MDK_STREAMS"] = item[1:]
+            if item[0] == "g":
+                os.environ["MDK_GRAPH"] = item[1:]
             bs = int(item[1:]) if item[0] == "b" else B
             sctx = M.Context(nf, ng, lig.na, t=t, max_batch=bs)
             sctx.set_receptor(gr, gi)
@@ -212,7 +219,9 @@ def main():
         "config": {"workload": "%s: synthetic receptor %d res (%d atoms) / ligand %d res (%d atoms), %d^3 FP32 complex grid, "
                                "%d rotations, top %d per rotation, %d decoys"
                                % (args.workload, cfg["rec_res"], rec.na, cfg["lig_res"], lig.na, nf, R, t, cfg["n_out"]),
-                   "nf": nf, "ng": ng, "rotations": R, "t": t, "batch": B},
+                   "nf": nf, "ng": ng, "rotations": R, "t": t, "batch": B,
+                   "variants": {"correlation": tuned[0], "inverse": tuned[1], "chosen_by": "autotune"} if tuned
+                   else "defaults"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic["bytes_per_batch"] if traffic else None,

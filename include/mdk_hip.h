@@ -112,6 +112,13 @@ int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float
  * rotation, i.e. Re IFFT(conj(R) * FFT(ligand)). */
 int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
 
+/* Autotuner: times every kernel variant of the specialised pipeline (correlation-pass
+ * tiling and receptor reuse, fused-inverse loading) on rotations [first, first+count) of
+ * the table and keeps the fastest for this context; choice[0..1] receives the chosen
+ * (correlation, inverse) variants.  All variants give bitwise-identical results.  No
+ * reference counterpart. */
+int mdk_autotune(mdk_ctx *ctx, int first, int count, int *choice);
+
 /* Off-grid score of B poses with its translation gradient, for refining decoys.  No
  * reference counterpart (extends the score of src/fft_process.cpp:333-446).  Pose b is
  * rotation rotcoef[9b..9b+8] (row-major, as mdk_dock) and a continuous translation

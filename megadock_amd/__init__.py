@@ -22,6 +22,7 @@ ABI_SYMBOLS = [
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
+    "mdk_autotune",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -177,6 +178,13 @@ class Context:
         gi = np.zeros(B * self.ng ** 3, np.float32)
         _check(lib().mdk_voxelize(self.h, _fp(rc), C.c_int(B), _fp(gr), _fp(gi)), "mdk_voxelize")
         return gr.reshape(B, -1), gi.reshape(B, -1)
+
+    def autotune(self, first, count):
+        """Time the kernel variants on rotations [first, first+count) of the table and keep
+        the fastest; returns the chosen (correlation, inverse) variant numbers."""
+        ch = (C.c_int * 2)()
+        _check(lib().mdk_autotune(self.h, C.c_int(first), C.c_int(count), ch), "mdk_autotune")
+        return ch[0], ch[1]
 
     def score_spectral(self, rotcoef, t):
         """(B, 4): band-limited score at continuous translations t (B, 3) in voxels and

@@ -283,11 +283,15 @@ __global__ void __launch_bounds__(NT) k_inv_z_top(const float2 *__restrict__ cc,
 // starting values: t == 1 starts from (0.0, index 0) and keeps a candidate only if its
 // value is > 0; t > 1 starts from -99999 and keeps candidates whose normalised value is
 // > -99999 (src/fft_process.cpp:407-446).  Scores are written normalised by nf^3.
+// boff (optional): device word with the batch's first rotation, added to rot_id and out
 __global__ void __launch_bounds__(NT) k_reduce_top(const Cand *__restrict__ cand, mdk_sortscore *__restrict__ out,
-                                                   const int *__restrict__ rot_id, int lists_per_rot,
-                                                   int t, int nf, float nf3f) {
+                                                   const int *__restrict__ rot_id, const int *__restrict__ boff,
+                                                   int lists_per_rot, int t, int nf, float nf3f) {
     __shared__ Cand lists[NT * MDK_MAXT];
     const int b = blockIdx.x;
+    const int b0 = boff ? *boff : 0;
+    rot_id += b0;
+    out += (size_t)b0 * t;
     const Cand *src = cand + (size_t)b * lists_per_rot * t;
     TopList tl;
     tl.init();
@@ -317,6 +321,24 @@ __global__ void __launch_bounds__(NT) k_reduce_top(const Cand *__restrict__ cand
             s.index[3] = idx % nf;
             out[(size_t)b * t + q] = s;
         }
+    }
+}
+
+// Batch cursor of a captured batch graph: ctl = {base, step, count}; every replay docks
+// rotations from base + count * step and advances count.
+__global__ void k_batch_begin(int *ctl, int *boff) {
+    if (threadIdx.x == 0) {
+        const int q = ctl[2];
+        boff[0] = ctl[0] + q * ctl[1];
+        ctl[2] = q + 1;
+    }
+}
+
+__global__ void k_batch_reset(int *ctl, int base, int step) {
+    if (threadIdx.x == 0) {
+        ctl[0] = base;
+        ctl[1] = step;
+        ctl[2] = 0;
     }
 }
 
@@ -401,6 +423,10 @@ struct BatchSlot {
     Cand *d_cand = nullptr;
     mdk_sortscore *d_bout = nullptr;  // max_batch * t
     hipEvent_t done = nullptr;        // recorded after a range's last batch on this stream
+    // one full batch captured as a graph (MDK_GRAPH): cursor, batch offset, executable
+    int *d_ctl = nullptr;             // {base, step, count}, then the batch offset
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct mdk_ctx {
@@ -429,6 +455,7 @@ struct mdk_ctx {
     // streams so that one batch's memory-bound passes overlap another's compute-bound ones
     BatchSlot slot[2];
     int nslots = 1;                // MDK_STREAMS (1 or 2)
+    bool use_graph = false;        // MDK_GRAPH: full batches of mdk_dock_range replay a captured graph
     mdk_sortscore *d_results = nullptr; // nrot * t
     // off-grid scores and gradients (mdk_grad.hip)
     float2 *d_recgrid = nullptr;  // receptor grids (grid_r, grid_i), ng^3, real space
@@ -508,18 +535,53 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
 }
 
 // ev (optional): MDK_NSTAGE + 1 events bracketing the stages of one batch
+// boff (optional): device word with the batch's first row, added to rot_dev, id_dev, out_dev
 int dock_batch(mdk_ctx *c, BatchSlot &S, const float *rot_dev, const int *id_dev, int B, mdk_sortscore *out_dev,
-               hipEvent_t *ev = nullptr) {
+               hipEvent_t *ev = nullptr, const int *boff = nullptr) {
     if (ev) HIPCHK(hipEventRecord(ev[0], S.stream));
-    HIPCHK(launch_voxelize(c->lig, rot_dev, B, S.vs, S.d_vidx, S.d_chg, S.stream));
+    HIPCHK(launch_voxelize(c->lig, rot_dev, B, S.vs, S.d_vidx, S.d_chg, S.stream, boff));
     if (ev) HIPCHK(hipEventRecord(ev[1], S.stream));
     if (run_fft_pipeline(c, S, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
     const int lists = c->use_pipe ? pipe_cands_per_rotation(nf) : nf * nf;
-    k_reduce_top<<<B, NT, 0, S.stream>>>(S.d_cand, out_dev, id_dev, lists, c->t, nf, (float)((size_t)nf * nf * nf));
+    k_reduce_top<<<B, NT, 0, S.stream>>>(S.d_cand, out_dev, id_dev, boff, lists, c->t, nf, (float)((size_t)nf * nf * nf));
     if (ev) HIPCHK(hipEventRecord(ev[7], S.stream));
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// Captures one full batch (max_batch rotations) of mdk_dock_range on slot S as a graph
+// whose replays take their first rotation from the slot's cursor (k_batch_begin).  The
+// graph bakes in the ligand, the rotation table, the results array and the kernel
+// variants, so set-up calls that change them drop it (drop_graphs).
+int ensure_graph(mdk_ctx *c, BatchSlot &S) {
+    if (S.exec) return 0;
+    if (!S.d_ctl) HIPCHK(hipMalloc((void **)&S.d_ctl, 4 * sizeof(int)));
+    HIPCHK(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    k_batch_begin<<<1, 64, 0, S.stream>>>(S.d_ctl, S.d_ctl + 3);
+    const int rc = dock_batch(c, S, c->d_rot, c->d_rotid, c->max_batch, c->d_results, nullptr, S.d_ctl + 3);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(S.stream, &g);
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(std::string("graph capture failed: ") + hipGetErrorString(e));
+    }
+    S.graph = g;
+    HIPCHK(hipGraphInstantiate(&S.exec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+void drop_graphs(mdk_ctx *c) {
+    for (int k = 0; k < 2; k++) {
+        BatchSlot &S = c->slot[k];
+        if (S.exec || S.graph) {
+            if (S.stream) (void)hipStreamSynchronize(S.stream);
+            if (S.exec) (void)hipGraphExecDestroy(S.exec);
+            if (S.graph) (void)hipGraphDestroy(S.graph);
+            S.exec = nullptr;
+            S.graph = nullptr;
+        }
+    }
 }
 
 template <typename T> int dmalloc(T **p, size_t n) {
@@ -591,6 +653,8 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     {
         const char *ns = std::getenv("MDK_STREAMS");
         c->nslots = (ns && std::atoi(ns) == 2) ? 2 : 1;  // two streams measured no faster at 144^3
+        const char *gr = std::getenv("MDK_GRAPH");
+        c->use_graph = gr && gr[0] == '1';
     }
     for (int k = 0; k < c->nslots; k++)
         if (hipStreamCreateWithFlags(&c->slot[k].stream, hipStreamNonBlocking) != hipSuccess ||
@@ -662,9 +726,10 @@ int mdk_destroy(mdk_ctx *c) {
                     c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    drop_graphs(c);
     for (int k = 0; k < 2; k++) {
         BatchSlot &S = c->slot[k];
-        void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout};
+        void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout, S.d_ctl};
         for (void *p : sp)
             if (p) (void)hipFree(p);
         if (S.done) (void)hipEventDestroy(S.done);
@@ -728,6 +793,7 @@ int mdk_set_ligand(mdk_ctx *c, const float *xyz, const float *center, const floa
     if (delta == 0.0f) return fail("mdk_set_ligand: core value 0 is not supported (the reference's last shape sweep is order-dependent then)");
     if (!(gw > 0.0f)) return fail("mdk_set_ligand: grid width must be positive");
     HIPCHK(hipSetDevice(c->device));
+    drop_graphs(c);  // they hold the ligand's kernel arguments
     const int na = c->na;
     std::vector<float> rc2(na), rs2(na);
     for (int i = 0; i < na; i++) {  // src/fft_process.cpp:784-787 / src/ligand.cpp:254-257
@@ -779,6 +845,7 @@ int mdk_dock(mdk_ctx *c, const float *rotcoef, const int *rot_id, int B, mdk_sor
 int mdk_set_rotations(mdk_ctx *c, const float *rotcoef, int R) {
     if (!c || !rotcoef || R < 1) return fail("mdk_set_rotations: invalid argument");
     HIPCHK(hipSetDevice(c->device));
+    drop_graphs(c);  // they hold the old table and results pointers
     if (c->d_rot) (void)hipFree(c->d_rot);
     if (c->d_rotid) (void)hipFree(c->d_rotid);
     if (c->d_results) (void)hipFree(c->d_results);
@@ -802,8 +869,20 @@ int mdk_dock_range(mdk_ctx *c, int first, int count) {
     // the other streams start after everything queued on the main stream so far
     HIPCHK(hipEventRecord(c->slot[0].done, c->stream));
     for (int k = 1; k < c->nslots; k++) HIPCHK(hipStreamWaitEvent(c->slot[k].stream, c->slot[0].done, 0));
-    int q = 0;
-    for (int b0 = first; b0 < first + count; b0 += c->max_batch, q++) {
+    int q = 0, b0 = first;
+    const int nfull = count / c->max_batch;
+    if (c->use_graph && nfull > 0) {  // full batches replay the captured graph of each slot
+        const int used = std::min(c->nslots, nfull);
+        for (int k = 0; k < used; k++) {
+            BatchSlot &S = c->slot[k];
+            if (ensure_graph(c, S)) return 1;
+            k_batch_reset<<<1, 64, 0, S.stream>>>(S.d_ctl, first + k * c->max_batch, c->nslots * c->max_batch);
+            HIPCHK(hipGetLastError());
+        }
+        for (; q < nfull; q++) HIPCHK(hipGraphLaunch(c->slot[q % c->nslots].exec, c->slot[q % c->nslots].stream));
+        b0 = first + nfull * c->max_batch;
+    }
+    for (; b0 < first + count; b0 += c->max_batch, q++) {
         const int B = std::min(c->max_batch, first + count - b0);
         if (dock_batch(c, c->slot[q % c->nslots], c->d_rot + 9 * (size_t)b0, c->d_rotid + b0, B,
                        c->d_results + (size_t)b0 * c->t))
@@ -924,6 +1003,40 @@ int mdk_score_map(mdk_ctx *c, const float *rotcoef, float *out) {
         rc = fail("mdk_score_map: copy back failed");
     (void)hipFree(d_map);
     return rc;
+}
+
+int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
+    if (!c) return fail("mdk_autotune: null context");
+    if (!c->have_rec || !c->have_lig || !c->d_rot) return fail("mdk_autotune: receptor, ligand and rotations must be set first");
+    if (first < 0 || count < 1 || first + count > c->nrot) return fail("mdk_autotune: range outside the rotation table");
+    HIPCHK(hipSetDevice(c->device));
+    if (c->use_pipe) {
+        // every variant computes the same results (same operations in the same order), so the
+        // fastest one can be chosen freely; each is timed twice after one warm-up pass
+        float best = 0.f;
+        int bx = c->xcorr_variant, bi = c->inv_variant;
+        for (int xv = 0; xv <= 4; xv++)
+            for (int iv = 0; iv <= 1; iv++) {
+                c->xcorr_variant = xv;
+                c->inv_variant = iv;
+                drop_graphs(c);
+                float ms = 0.f;
+                if (mdk_dock_range(c, first, count) || mdk_time_range(c, first, count, 2, &ms)) return 1;
+                if (best == 0.f || ms < best) {
+                    best = ms;
+                    bx = xv;
+                    bi = iv;
+                }
+            }
+        c->xcorr_variant = bx;
+        c->inv_variant = bi;
+        drop_graphs(c);
+    }
+    if (choice) {
+        choice[0] = c->xcorr_variant;
+        choice[1] = c->inv_variant;
+    }
+    return 0;
 }
 
 int mdk_score_spectral(mdk_ctx *c, const float *rotcoef, const float *t, int B, float *out) {

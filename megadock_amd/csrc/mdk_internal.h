@@ -60,8 +60,9 @@ __device__ __forceinline__ float2 vox_take(const VoxGrid &g, int b, int row, int
 
 // Ligand voxelization of B rotations (rot: B*9 rotation coefficients, row-major r11..r33)
 // into the masks of vs and the charge grid chg (B * ng^3, zero on entry); vidx: B*na ints.
+// boff (optional): device word with the batch's first row of rot (graph-captured batches)
 hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const VoxScratch &vs, int *vidx,
-                           float *chg, hipStream_t s);
+                           float *chg, hipStream_t s, const int *boff = nullptr);
 // (grid_r, grid_i) pairs lg[b*ng^3 + v] from a voxelized batch; clears its charge grid
 hipError_t launch_vox_expand(const VoxGrid &vg, int B, float2 *lg, hipStream_t s);
 

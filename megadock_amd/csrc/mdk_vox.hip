@@ -56,14 +56,16 @@ __device__ __forceinline__ void or_run(uint32_t *m, int row, int k0, int k1) {
 }
 
 // one thread per (atom, rotation): core/surface bits, the charge voxel and its atom count
-__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, uint32_t *cs, uint32_t *cnt, int *vidx,
-                            int mw, int cw) {
+// boff (optional): device word holding the batch's first row of rot
+__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, const int *__restrict__ boff, uint32_t *cs,
+                            uint32_t *cnt, int *vidx, int mw, int cw) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (l >= L.na) return;
+    const float *rb = rot + 9 * ((size_t)(boff ? *boff : 0) + b);
     float r[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) r[q] = rot[9 * b + q];
+    for (int q = 0; q < 9; q++) r[q] = rb[q];
     float X, Y, Z;
     rotate_atom(L, r, l, X, Y, Z);
     const int ng = L.ng, rw = row_words(ng);
@@ -240,12 +242,12 @@ __global__ void k_vox_expand(VoxGrid vg, float2 *lg, int ng3) {
 }  // namespace
 
 hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const VoxScratch &vs, int *vidx,
-                           float *chg, hipStream_t s) {
+                           float *chg, hipStream_t s, const int *boff) {
     const int ng = lig.ng, mw = mask_words(ng), cw = count_words(ng);
     const int ng3 = ng * ng * ng;
     const size_t nclear = 2 * (size_t)B * mw;  // core and surface bits; counts are kept zero
     k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(vs.cs, nclear);
-    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, vs.cs, vs.cnt, vidx, mw, cw);
+    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, boff, vs.cs, vs.cnt, vidx, mw, cw);
     k_vox_masks<<<dim3((mw + 255) / 256, B), 256, 0, s>>>(vs.cs, vs.dmo, ng, mw, lig.delta);
     k_vox_charge<<<B, kChargeThreads, 0, s>>>(vidx, vs.cnt, lig.charge, chg, lig.na, ng3, cw);
     return hipGetLastError();

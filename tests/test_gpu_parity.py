@@ -314,3 +314,72 @@ def test_tuning_variants_give_identical_results(gpu, gcq, env):
             else:
                 os.environ[k] = v
     assert np.array_equal(base.view(np.uint8), other.view(np.uint8))
+
+
+def test_autotune_keeps_results(gpu, gcq):
+    rec, lig, _ = gcq
+    ng = 48
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rc = M.rotation_coefs(O.load_rotations(3600)[:256])
+    ctx = _lig_ctx(lig, ng, t=2, max_batch=64)
+    ctx.set_receptor(gr, gi)
+    ctx.set_rotations(rc)
+    ctx.dock_range(0, 256)
+    ctx.sync()
+    before = ctx.results(0, 256)
+    xv, iv = ctx.autotune(0, 128)
+    _say("autotune chose correlation variant %d, inverse variant %d" % (xv, iv))
+    assert 0 <= xv <= 4 and 0 <= iv <= 1
+    ctx.dock_range(0, 256)
+    ctx.sync()
+    assert np.array_equal(before.view(np.uint8), ctx.results(0, 256).view(np.uint8))
+    ctx.close()
+
+
+def test_graph_replays_match_eager(gpu, gcq):
+    """MDK_GRAPH=1: the full batches of mdk_dock_range replay a captured graph.  Three
+    ranges in a row, with eager batches (the tails) and synchronisations between the
+    replays, each checked against the eager path; then a new rotation table (the graph is
+    captured again)."""
+    rec, lig, _ = gcq
+    ng = 48
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ang = O.load_rotations(3600)
+    rc1, rc2 = M.rotation_coefs(ang[:400]), M.rotation_coefs(ang[1000:1300])
+
+    def make():
+        ctx = _lig_ctx(lig, ng, t=3, max_batch=64)
+        ctx.set_receptor(gr, gi)
+        return ctx
+
+    eager = make()
+    eager.set_rotations(rc1)
+    eager.dock_range(0, 400)
+    eager.sync()
+    ref1 = eager.results(0, 400)
+    eager.set_rotations(rc2)
+    eager.dock_range(0, 300)
+    eager.sync()
+    ref2 = eager.results(0, 300)
+    eager.close()
+    old = os.environ.get("MDK_GRAPH")
+    os.environ["MDK_GRAPH"] = "1"
+    try:
+        g = make()
+    finally:
+        if old is None:
+            os.environ.pop("MDK_GRAPH", None)
+        else:
+            os.environ["MDK_GRAPH"] = old
+    g.set_rotations(rc1)
+    for first, count in [(0, 400), (64, 200), (7, 333)]:
+        _say("graph replay range %d+%d" % (first, count))
+        g.dock_range(first, count)
+        g.sync()
+        got = g.results(first, count)
+        assert np.array_equal(got.view(np.uint8), ref1[first * 3:(first + count) * 3].view(np.uint8))
+    g.set_rotations(rc2)
+    g.dock_range(0, 300)
+    g.sync()
+    assert np.array_equal(g.results(0, 300).view(np.uint8), ref2.view(np.uint8))
+    g.close()
