@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
     ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i), "
-                    "stream counts (s), graph replay on/off (g) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
+                    "stream counts (s), graph replay on/off (g), forward-pass variants (f) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
                     "one stage-timing line per setting and exits")
     args = ap.parse_args()
 
@@ -136,6 +136,8 @@ def main():
                 os.environ["MDK_STREAMS"] = item[1:]
             if item[0] == "g":
                 os.environ["MDK_GRAPH"] = item[1:]
+            if item[0] == "f":
+                os.environ["MDK_FWD_VARIANT"] = item[1:]
             bs = int(item[1:]) if item[0] == "b" else B
             sctx = M.Context(nf, ng, lig.na, t=t, max_batch=bs)
             sctx.set_receptor(gr, gi)

@@ -440,6 +440,7 @@ struct mdk_ctx {
     int xcorr_variant = 4;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
     int cu_count = 256;       // compute units (persistent grids)
     int inv_variant = 0;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
+    int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, 1 separate passes
     float2 *d_rec = nullptr;
     bool have_rec = false, have_lig = false;
     // ligand
@@ -511,6 +512,7 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
         a.variant = c->xcorr_variant;
         a.cu_count = c->cu_count;
         a.inv_variant = c->inv_variant;
+        a.fwd_variant = c->fwd_variant;
         a.spectrum_only = spectrum_only;
         HIPCHK(pipe_run(nf, a));
         return 0;
@@ -698,6 +700,8 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         c->xcorr_variant = xv ? std::atoi(xv) : 4;
         const char *iv = std::getenv("MDK_INV_VARIANT");
         c->inv_variant = iv ? std::atoi(iv) : 0;
+        const char *fv = std::getenv("MDK_FWD_VARIANT");
+        c->fwd_variant = fv ? std::atoi(fv) : 0;
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
             c->cu_count = cus;

@@ -262,6 +262,7 @@ struct PipeArgs {
     int cu_count;       // compute units of the device (persistent grids)
     int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches (MDK_INV_VARIANT)
     bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
+    int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, 1 separate (MDK_FWD_VARIANT)
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.

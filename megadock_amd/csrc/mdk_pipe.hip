@@ -101,6 +101,89 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
     }
 }
 
+// ------------------------------------------------------------------ forward z + y, fused
+// Persistent workgroups walking the (b, x) planes, x < ng, of the ligand.  Per plane:
+//   z-FFT of the ng rows y (voxel values straight from the masks and the charge grid into
+//   the step-1 registers), kept in LDS as A[kz][y];
+//   y-FFT of the nf columns kz in chunks of C, the ng non-zero rows at y + nd read from A,
+//   the outputs written straight to a2[b][x][ky][kz].
+// It replaces kp_fwd_z + kp_fwd_y and never writes a1.
+template <int N, int C> constexpr bool fwd_zy_fits() {
+    constexpr int NG = N / 2, T = Geo<N>::T;
+    return NG * T <= 1024 && C * T <= 1024 && N % C == 0 &&
+           (size_t)N * (NG + 1) * 8 + (size_t)N * (C + 1) * 8 + (size_t)N * 8 <= 150 * 1024;
+}
+
+template <int N, int C>
+__global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict__ a2, const float2 *__restrict__ twg,
+                                                   int nplanes) {
+    constexpr int NG = N / 2, ND = N / 4;
+    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
+    constexpr int LA = NG + 1, LE = C + 1;
+    static_assert(fwd_zy_fits<N, C>(), "fused forward pass does not fit");
+    __shared__ float2 A[N * LA];  // z-transformed plane, [kz][y]
+    __shared__ float2 E[N * LE];  // y-exchange of one chunk of columns
+    __shared__ float2 tw[N];
+    for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
+    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
+        const int b = p / NG, x = p - b * NG;
+        // opaque thread index: keeps the per-thread addresses out of the plane loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        __syncthreads();  // A, E (and tw) free
+        // ---- z-FFT of row y: thread (y, u), u fastest so that a row's voxels sit in neighbouring lanes
+        {
+            const int y = tid / T, u = tid - y * T;
+            const bool zrow = y < NG;
+            const int row = x * NG + y;
+            if (zrow && u < N2) {
+                float2 v[N1] = {};
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    const int e = u + N2 * j1;
+                    if (e >= ND && e < ND + NG) v[j1] = vox_take(vg, b, row, e - ND);
+                });
+                rf::step1_g<N, false>(v, A, tw, 1, LA, 1, y, u);
+            }
+            __syncthreads();
+            float2 w[N2] = {};
+            if (zrow && u < N1) rf::step2_g<N, false>(w, A, LA, 1, y, u);
+            __syncthreads();
+            if (zrow && u < N1)
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    A[(u + N1 * k2) * LA + y] = w[k2];
+                });
+        }
+        __syncthreads();
+        // ---- y-FFT of the columns, C at a time: thread (column, u), column fastest
+        float2 *dst = a2 + (size_t)p * N * N;  // a2[b][x]: [ky][kz]
+        const int cl = tid % C, u = tid / C;
+        for (int c0 = 0; c0 < N; c0 += C) {  // uniform
+            const int kz = c0 + cl;
+            if (u < N2) {
+                float2 v[N1] = {};
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    const int yy = u + N2 * j1 - ND;
+                    if (yy >= 0 && yy < NG) v[j1] = A[kz * LA + yy];
+                });
+                rf::step1_g<N, false>(v, E, tw, 1, LE, 1, cl, u);
+            }
+            __syncthreads();
+            if (u < N1) {
+                float2 w[N2];
+                rf::step2_g<N, false>(w, E, LE, 1, cl, u);
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    st_at(dst, 8u * (unsigned)((u + N1 * k2) * N + kz), w[k2]);
+                });
+            }
+            __syncthreads();  // E is rewritten by the next chunk
+        }
+    }
+}
+
 // ------------------------------------------------------------------ x + correlation
 // block (kz chunk c, ky, rotation group).  Lines 0..L-1: (ky, kz = c L + l); lines
 // L..2L-1: partner columns (-ky, -kz).  Forward x-FFT of all 2L lines, the Hermitian
@@ -544,16 +627,35 @@ template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
         kp_inv_yz_top<N, TT, false><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
 }
 
+// columns per chunk of the fused forward pass, or 0 where it does not fit (then z and y
+// run as two passes)
+constexpr int fwd_zy_chunk(int n) {
+    return n == 80 ? 80 : n == 96 ? 48 : n == 144 ? 48 : 0;
+}
+
 template <int N> hipError_t run_pipe(const PipeArgs &a) {
     constexpr int T = Geo<N>::T;
     constexpr int L = 16, LX = 8;
     const int B = a.B, ng = a.ng;
     if (ng != N / 2 || a.nd != N / 4) return hipErrorInvalidValue;  // the kernels assume nf = 2 ng, nd = ng / 2
     hipStream_t s = a.stream;
-    const int lines_z = B * ng * ng;
-    kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.vox, a.a1, a.tw4, lines_z);
-    if (a.ev) (void)hipEventRecord(a.ev[0], s);
-    kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
+    constexpr int CZY = fwd_zy_chunk(N);
+    bool fused_fwd = false;
+    if constexpr (CZY > 0) {
+        if (a.fwd_variant == 0) {  // fused z + y
+            const int nplanes = B * ng;
+            const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;
+            kp_fwd_zy<N, CZY><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+            if (a.ev) (void)hipEventRecord(a.ev[0], s);  // the z stage is fused into the y stage
+            fused_fwd = true;
+        }
+    }
+    if (!fused_fwd) {
+        const int lines_z = B * ng * ng;
+        kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.vox, a.a1, a.tw4, lines_z);
+        if (a.ev) (void)hipEventRecord(a.ev[0], s);
+        kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
+    }
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
     if (a.spectrum_only) {
         kp_x_corr<N, LX, 1, false, true, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,

@@ -281,7 +281,7 @@ def test_topn_device_merge_all_tied(gpu, gcq):
 
 
 @pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "0123"] +
-                         [{"MDK_INV_VARIANT": "1"}, {"MDK_STREAMS": "2"}])
+                         [{"MDK_INV_VARIANT": "1"}, {"MDK_STREAMS": "2"}, {"MDK_FWD_VARIANT": "1"}])
 def test_tuning_variants_give_identical_results(gpu, gcq, env):
     """Every kernel variant computes the same operations in the same order: results are
     bitwise equal to the default's."""
