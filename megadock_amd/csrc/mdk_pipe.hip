@@ -664,7 +664,8 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     }
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
     // the grid, so consecutive workgroups share the receptor tile in L2, folded lines kept
-    // in the exchange buffer's LDS (4, the default); the same with a separate fold buffer
+    // in the exchange buffer's LDS, 16 columns per workgroup (5, the default) or 8 (4); 8
+    // columns with a separate fold buffer
     // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
     // of 4 rotations (1); one rotation per workgroup, tile index fastest (2)
     switch (a.variant) {
@@ -682,9 +683,13 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
         case 3:
             kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
-        default:
+        case 4:
             kp_x_corr<N, LX, 1, false, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
                                                                                               a.H, a.Hp, B);
+            break;
+        default:  // 5: as 4 with 16 columns per workgroup
+            kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
     }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     if constexpr (fused_inverse(N)) {
