@@ -137,12 +137,31 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
             const bool zrow = y < NG;
             const int row = x * NG + y;
             if (zrow && u < N2) {
+                // the row's mask words once, then one bit per voxel; the charges one by one
+                constexpr int RW = (NG + 31) / 32;
+                static_assert(RW <= 4, "row of at most 128 voxels");
+                const uint32_t *dr = vg.dmo + (size_t)b * 2 * vg.mw + (size_t)row * RW;
+                const int mw = vg.mw;
+                const uint32_t d0 = dr[0], o0 = dr[mw];
+                const uint32_t d1 = RW > 1 ? dr[1] : 0u, o1 = RW > 1 ? dr[mw + 1] : 0u;
+                const uint32_t d2 = RW > 2 ? dr[2] : 0u, o2 = RW > 2 ? dr[mw + 2] : 0u;
+                const uint32_t d3 = RW > 3 ? dr[3] : 0u, o3 = RW > 3 ? dr[mw + 3] : 0u;
+                float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)row * NG;
+                const float delta = vg.delta;
                 float2 v[N1] = {};
-                sfor<0, N1>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
+#pragma unroll
+                for (int j1 = 0; j1 < N1; j1++) {
                     const int e = u + N2 * j1;
-                    if (e >= ND && e < ND + NG) v[j1] = vox_take(vg, b, row, e - ND);
-                });
+                    if (e >= ND && e < ND + NG) {
+                        const int k = e - ND, w = k >> 5;
+                        const uint32_t d = w == 0 ? d0 : w == 1 ? d1 : w == 2 ? d2 : d3;
+                        const uint32_t o = w == 0 ? o0 : w == 1 ? o1 : w == 2 ? o2 : o3;
+                        const uint32_t bit = 1u << (k & 31);
+                        const float q = cr[k];
+                        if (__float_as_uint(q) != 0u) cr[k] = 0.0f;  // keep the charge grid zero
+                        v[j1] = make_float2((d & bit) ? delta : ((o & bit) ? 1.0f : 0.0f), q);
+                    }
+                }
                 rf::step1_g<N, false>(v, A, tw, 1, LA, 1, y, u);
             }
             __syncthreads();
