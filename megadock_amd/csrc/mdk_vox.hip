@@ -55,17 +55,10 @@ __device__ __forceinline__ void or_run(uint32_t *m, int row, int k0, int k1) {
     }
 }
 
-// one thread per (atom, rotation): core/surface bits, the charge voxel and its atom count
-// boff (optional): device word holding the batch's first row of rot
-__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, const int *__restrict__ boff, uint32_t *cs,
-                            uint32_t *cnt, int *vidx, int mw, int cw) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (l >= L.na) return;
-    const float *rb = rot + 9 * ((size_t)(boff ? *boff : 0) + b);
-    float r[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) r[q] = rb[q];
+// Atom l of rotation b: its core/surface bit runs into cb/sb (global or LDS masks of one
+// rotation), its charge voxel into vidx and the voxel's atom count.
+__device__ __forceinline__ void vox_atom(const LigandDev &L, const float *r, int l, int b, uint32_t *cb,
+                                         uint32_t *sb, uint32_t *cnt, int *vidx, int cw) {
     float X, Y, Z;
     rotate_atom(L, r, l, X, Y, Z);
     const int ng = L.ng, rw = row_words(ng);
@@ -76,8 +69,6 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, const in
     const int ia = max(i2 - 2, 0), ja = max(j2 - 2, 0), ka = max(k2 - 2, 0);
     const int ib = min(i2 + 3, ng), jb = min(j2 + 3, ng), kb = min(k2 + 3, ng);
     const float rc2 = L.rc2[l], rs2 = L.rs2[l];
-    uint32_t *cb = cs + (size_t)b * 2 * mw;  // core bits
-    uint32_t *sb = cb + mw;                  // surface bits
     for (int i = ia; i < ib; i++) {
         const float dx = X - L.gc[i];
         const float xd = dx * dx;
@@ -106,6 +97,21 @@ __global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, const in
     const int v = (ci * ng + cj) * ng + ck;
     vidx[(size_t)b * L.na + l] = v;
     atomicAdd(&cnt[(size_t)b * cw + (v >> 1)], 1u << (16 * (v & 1)));
+}
+
+// one thread per (atom, rotation): core/surface bits, the charge voxel and its atom count
+// boff (optional): device word holding the batch's first row of rot
+__global__ void k_vox_atoms(LigandDev L, const float *__restrict__ rot, const int *__restrict__ boff, uint32_t *cs,
+                            uint32_t *cnt, int *vidx, int mw, int cw) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (l >= L.na) return;
+    const float *rb = rot + 9 * ((size_t)(boff ? *boff : 0) + b);
+    float r[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) r[q] = rb[q];
+    uint32_t *cb = cs + (size_t)b * 2 * mw;  // core bits
+    vox_atom(L, r, l, b, cb, cb + mw, cnt, vidx, cw);
 }
 
 // bits k = 32 w + s of word w with lo <= k <= hi
@@ -137,14 +143,12 @@ __device__ __forceinline__ uint32_t core2_word(const MaskRows &C, int i, int j, 
     return c & ~(range_bits(w, 1, ng - 2) & ~closed);
 }
 
-// one thread per (row word, rotation): the delta mask D and the value-1 mask O
-__global__ void k_vox_masks(const uint32_t *__restrict__ cs, uint32_t *dmo, int ng, int mw, float delta) {
-    const int rw = row_words(ng);
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = blockIdx.y;
-    if (q >= mw) return;
+// word q of the delta mask D and the value-1 mask O of one rotation from its core (C) and
+// surface (S) bits
+__device__ __forceinline__ void vox_mask_word(const MaskRows &C, const MaskRows &S, int q, int ng, float delta,
+                                              uint32_t &dq, uint32_t &oq) {
+    const int rw = C.rw;
     const int w = q % rw, row = q / rw, i = row / ng, j = row - i * ng;
-    const MaskRows C{cs + (size_t)b * 2 * mw, ng, rw}, S{cs + (size_t)b * 2 * mw + mw, ng, rw};
     uint32_t d = core2_word(C, i, j, w);
     uint32_t o = ~d & S.at(i, j, w);
     if (delta != 1.0f && i >= 1 && i <= ng - 2 && j >= 1 && j <= ng - 2) {
@@ -158,8 +162,49 @@ __global__ void k_vox_masks(const uint32_t *__restrict__ cs, uint32_t *dmo, int 
         d &= ~to1;
         o |= to1;
     }
+    dq = d;
+    oq = o;
+}
+
+// one thread per (row word, rotation): the delta mask D and the value-1 mask O
+__global__ void k_vox_masks(const uint32_t *__restrict__ cs, uint32_t *dmo, int ng, int mw, float delta) {
+    const int rw = row_words(ng);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (q >= mw) return;
+    const MaskRows C{cs + (size_t)b * 2 * mw, ng, rw}, S{cs + (size_t)b * 2 * mw + mw, ng, rw};
+    uint32_t d, o;
+    vox_mask_word(C, S, q, ng, delta, d, o);
     dmo[(size_t)b * 2 * mw + q] = d;
     dmo[(size_t)b * 2 * mw + mw + q] = o;
+}
+
+// One workgroup per rotation with its core and surface bits in LDS (grids whose masks
+// fit): atom runs by LDS atomics, then the D and O words straight to dmo.  Replaces
+// k_clear + k_vox_atoms + k_vox_masks where it fits.
+constexpr int kVoxLdsThreads = 1024;
+constexpr size_t kVoxLdsBytes = 132 * 1024;  // both masks of one rotation, at most
+__global__ void __launch_bounds__(kVoxLdsThreads)
+    k_vox_lds(LigandDev L, const float *__restrict__ rot, const int *__restrict__ boff, uint32_t *dmo, uint32_t *cnt,
+              int *vidx, int mw, int cw) {
+    extern __shared__ uint32_t msk[];  // [2][mw]: core, surface
+    const int b = blockIdx.x, tid = threadIdx.x, ng = L.ng;
+    for (int q = tid; q < 2 * mw; q += kVoxLdsThreads) msk[q] = 0u;
+    const float *rb = rot + 9 * ((size_t)(boff ? *boff : 0) + b);
+    float r[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) r[q] = rb[q];
+    __syncthreads();
+    for (int l = tid; l < L.na; l += kVoxLdsThreads) vox_atom(L, r, l, b, msk, msk + mw, cnt, vidx, cw);
+    __syncthreads();
+    const MaskRows C{msk, ng, row_words(ng)}, S{msk + mw, ng, row_words(ng)};
+    uint32_t *db = dmo + (size_t)b * 2 * mw;
+    for (int q = tid; q < mw; q += kVoxLdsThreads) {
+        uint32_t d, o;
+        vox_mask_word(C, S, q, ng, L.delta, d, o);
+        db[q] = d;
+        db[mw + q] = o;
+    }
 }
 
 // Charges of the atoms nearest to each voxel, summed in atom order from +0.0f
@@ -245,10 +290,16 @@ hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const 
                            float *chg, hipStream_t s, const int *boff) {
     const int ng = lig.ng, mw = mask_words(ng), cw = count_words(ng);
     const int ng3 = ng * ng * ng;
-    const size_t nclear = 2 * (size_t)B * mw;  // core and surface bits; counts are kept zero
-    k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(vs.cs, nclear);
-    k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, boff, vs.cs, vs.cnt, vidx, mw, cw);
-    k_vox_masks<<<dim3((mw + 255) / 256, B), 256, 0, s>>>(vs.cs, vs.dmo, ng, mw, lig.delta);
+    const size_t lds = 2 * sizeof(uint32_t) * (size_t)mw;
+    if (lds <= kVoxLdsBytes) {  // one workgroup per rotation, bits in LDS
+        k_vox_lds<<<B, kVoxLdsThreads, lds, s>>>(lig, rot, boff, vs.dmo, vs.cnt, vidx, mw, cw);
+    } else {
+        const size_t nclear = 2 * (size_t)B * mw;  // core and surface bits; counts are kept zero
+        k_clear<<<dim3((unsigned)((nclear + 255) / 256 < 65535 ? (nclear + 255) / 256 : 65535)), 256, 0, s>>>(vs.cs,
+                                                                                                       nclear);
+        k_vox_atoms<<<dim3((lig.na + 127) / 128, B), 128, 0, s>>>(lig, rot, boff, vs.cs, vs.cnt, vidx, mw, cw);
+        k_vox_masks<<<dim3((mw + 255) / 256, B), 256, 0, s>>>(vs.cs, vs.dmo, ng, mw, lig.delta);
+    }
     k_vox_charge<<<B, kChargeThreads, 0, s>>>(vidx, vs.cnt, lig.charge, chg, lig.na, ng3, cw);
     return hipGetLastError();
 }

@@ -95,10 +95,11 @@ def test_voxelize_bit_exact_1gcq(gpu, gcq):
     ctx.close()
 
 
-@pytest.mark.parametrize("delta,ng", [(2.0, None), (1.0, 24), (-3.5, 30)])
+@pytest.mark.parametrize("delta,ng", [(2.0, None), (1.0, 24), (-3.5, 30), (2.0, 80), (1.0, 88)])
 def test_voxelize_bit_exact_edge(gpu, gcq, delta, ng):
-    """core value != 1 (the extra surface sweep) and grids too small for the ligand
-    (atoms outside the box: clamped charges, truncated search boxes)."""
+    """core value != 1 (the extra surface sweep), grids too small for the ligand (atoms
+    outside the box: clamped charges, truncated search boxes) and grids whose bit masks
+    exceed one workgroup's LDS (ng 80, 88: the global-memory voxelizer)."""
     rec, lig, ng0 = gcq
     ng = ng or ng0
     _say("voxelize edge delta=%g ng=%d" % (delta, ng))
