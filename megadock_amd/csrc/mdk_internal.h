@@ -260,7 +260,8 @@ struct PipeArgs {
     hipEvent_t *ev;     // optional: 5 events, one after each pass
     int variant;        // kernel variant of the correlation pass (MDK_XCORR_VARIANT)
     int cu_count;       // compute units of the device (persistent grids)
-    int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches (MDK_INV_VARIANT)
+    int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches, 2 and 3 as 0 and 1
+                        // with running top-t lists over contiguous planes (MDK_INV_VARIANT)
     bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
     int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, 1 separate (MDK_FWD_VARIANT)
 };

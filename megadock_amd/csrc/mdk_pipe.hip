@@ -493,9 +493,14 @@ constexpr bool fused_inverse(int n) { return (size_t)n * plane_stride(n) * 8 <= 
 //     Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])),  k < M,
 //   whose M-point inverse gives z[m] = x[2m] + i x[2m+1] (x the real score row);
 //   the M-point inverse of each row, whose outputs go straight into the top-t lists.
-// The plane's top-t is written as one candidate list per plane.  PF: prefetch the next
-// plane's step-1 inputs (in flight during the current plane) or load them at the start.
-template <int N, int TT, bool PF>
+// The top-t is written as one candidate list per plane.  PF: prefetch the next plane's
+// step-1 inputs (in flight during the current plane) or load them at the start.
+// SEG: each workgroup walks a contiguous range of planes instead; the planes of one
+// rotation inside the range form a segment whose lane lists, and whose running threshold,
+// carry over from plane to plane.  The segment's top-t goes to the list of its first plane
+// and the lists of its other planes are left empty, so the wavefront merge and the
+// block merge run once per segment rather than once per plane.
+template <int N, int TT, bool PF, bool SEG = false>
 __global__ void __launch_bounds__(1024)
     kp_inv_yz_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                   const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
@@ -512,9 +517,17 @@ __global__ void __launch_bounds__(1024)
     const int tid = threadIdx.x;
     for (int q = tid; q < N; q += NTH) tw[q] = twg[q];
     // y task of a thread: column kz = yl, step-1 index yu (< N2y), step-2 index yu (< N1y)
+    // planes of this workgroup: p_first, p_first + p_step, ... below p_end
+    int p_first = blockIdx.x, p_end = nplanes, p_step = gridDim.x;
+    if constexpr (SEG) {
+        const int chunk = (nplanes + gridDim.x - 1) / gridDim.x;
+        p_first = blockIdx.x * chunk;
+        p_end = min(nplanes, p_first + chunk);
+        p_step = 1;
+    }
     float2 vn[N1y];  // step-1 inputs of the plane to come
     auto load_plane = [&](int p, int yl, int yu) {
-        if (yu < N2y && p < nplanes) {
+        if (yu < N2y && p < p_end) {
             const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
             const unsigned off = (unsigned)(yu * Hp + yl);
             sfor<0, N1y>([&](auto jc) {
@@ -523,8 +536,14 @@ __global__ void __launch_bounds__(1024)
             });
         }
     };
-    if (PF) load_plane(blockIdx.x, tid % H, tid / H);
-    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
+    constexpr bool QUOT = TT > 1;
+    // SEG: the running lane lists, the running threshold and the segment's first plane
+    TopR<TT> tl;
+    tl.init();
+    float run_tau = -INFINITY, cut = -INFINITY;
+    int seg0 = p_first;
+    if (PF) load_plane(p_first, tid % H, tid / H);
+    for (int p = p_first; p < p_end; p += p_step) {  // uniform across the block
         const int b = p / N, x = p - b * N;
         // The thread's LDS addresses are recomputed in every iteration: an opaque copy of
         // the thread index keeps them from being hoisted out of the plane loop, where they
@@ -540,7 +559,7 @@ __global__ void __launch_bounds__(1024)
             sfor<0, N1y>([&](auto jc) { v[decltype(jc)::value] = vn[decltype(jc)::value]; });
             rf::step1_g<N, true>(v, P, tw, 1, LSP, 1, yl, yu);
         }
-        if (PF) load_plane(p + gridDim.x, yl, yu);
+        if (PF) load_plane(p + p_step, yl, yu);
         __syncthreads();
         {
             float2 y[N2y];
@@ -606,20 +625,27 @@ __global__ void __launch_bounds__(1024)
         // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
         // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
         // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
-        constexpr bool QUOT = TT > 1;
-        float mx = -INFINITY;
+        // SEG: tau is the largest such bound over the segment's planes so far, taken from
+        // its first four planes and every eighth after; the t scores behind it were pushed
+        // when it was raised and can only be displaced by better ones.
+        const int k = p - seg0;
+        if (!SEG || k < 4 || (k & 7) == 0) {  // uniform
+            float mx = -INFINITY;
 #pragma unroll
-        for (int j = 0; j < KZ; j++) {
-            if ((tid + j * NTH) / N < N1z)
-                sfor<0, N2z>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
-                });
+            for (int j = 0; j < KZ; j++) {
+                if ((tid + j * NTH) / N < N1z)
+                    sfor<0, N2z>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
+                    });
+            }
+            const float tau = wave_kth_max(mx, t);
+            if (!SEG || tau > run_tau) {  // uniform
+                run_tau = tau;
+                cut = QUOT ? quotient_cut(tau, nf3f) : tau;
+            }
         }
-        const float tau = wave_kth_max(mx, t);
-        const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
-        TopR<TT> tl;
-        tl.init();
+        if (!SEG) tl.init();
 #pragma unroll
         for (int j = 0; j < KZ; j++) {
             const int task = tid + j * NTH, line = task % N, u = task / N;
@@ -633,17 +659,38 @@ __global__ void __launch_bounds__(1024)
                 });
             }
         }
-        block_top<TT, NW>(tl, t, wl, cand + (size_t)p * t);
+        if constexpr (SEG) {
+            if (p != seg0 && tid < t) cand[(size_t)p * t + tid] = Cand{-INFINITY, kNoIndex};  // empty list
+            if (p + 1 == p_end || (p + 1) / N != b) {  // uniform: the segment ends with this plane
+                block_top<TT, NW>(tl, t, wl, cand + (size_t)seg0 * t);
+                tl.init();
+                run_tau = cut = -INFINITY;
+                seg0 = p + 1;
+            }
+        } else {
+            block_top<TT, NW>(tl, t, wl, cand + (size_t)p * t);
+        }
     }
 }
 
 template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
     const int nplanes = a.B * N;
     const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
-    if (a.inv_variant == 1)
-        kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
-    else
-        kp_inv_yz_top<N, TT, false><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    switch (a.inv_variant) {
+        case 1:
+            kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+            break;
+        case 2:
+            kp_inv_yz_top<N, TT, false, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f,
+                                                                  nplanes);
+            break;
+        case 3:
+            kp_inv_yz_top<N, TT, true, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f,
+                                                                 nplanes);
+            break;
+        default:
+            kp_inv_yz_top<N, TT, false><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    }
 }
 
 // columns per chunk of the fused forward pass, or 0 where it does not fit (then z and y

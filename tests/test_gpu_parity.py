@@ -281,8 +281,8 @@ def test_topn_device_merge_all_tied(gpu, gcq):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "01234"] +
-                         [{"MDK_INV_VARIANT": "1"}, {"MDK_STREAMS": "2"}, {"MDK_FWD_VARIANT": "1"}])
+@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "012345"] +
+                         [{"MDK_INV_VARIANT": v} for v in "123"] + [{"MDK_STREAMS": "2"}, {"MDK_FWD_VARIANT": "1"}])
 def test_tuning_variants_give_identical_results(gpu, gcq, env):
     """Every kernel variant computes the same operations in the same order: results are
     bitwise equal to the default's."""
@@ -317,6 +317,57 @@ def test_tuning_variants_give_identical_results(gpu, gcq, env):
     assert np.array_equal(base.view(np.uint8), other.view(np.uint8))
 
 
+def _with_env(env, fn):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return fn()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("ng,t", [(48, 1), (48, 2), (48, 4), (40, 7), (72, 3), (48, 16)])
+@pytest.mark.parametrize("tied", [False, True])
+def test_segmented_top_t_lists(gpu, gcq, ng, t, tied):
+    """Fused-inverse variants 2 and 3 keep running top-t lists over the planes of a rotation
+    that one workgroup walks (segments, some cut short where a workgroup's range of planes
+    ends inside a rotation); per-rotation results must be bitwise those of the per-plane
+    lists (variant 0), for t over every compile-time list length, for a batch whose plane
+    count does not split evenly, and with every score tied."""
+    rec, lig, _ = gcq
+    nf = 2 * ng
+    ang = O.load_rotations(3600)[:150]
+    rc = M.rotation_coefs(ang)
+    if tied:
+        z = np.zeros(nf ** 3, np.float32)
+
+    def run():
+        ctx = _lig_ctx(lig, ng, t=t, max_batch=53)  # batches of 53, 53 and 44 rotations
+        if tied:
+            ctx.set_receptor_spectrum(z, z)
+        else:
+            ctx.set_receptor(*O.receptor_voxel(ng, 1.2, rec))
+        ctx.set_rotations(rc)
+        ctx.dock_range(0, len(rc))
+        ctx.sync()
+        out = ctx.results(0, len(rc))
+        ctx.close()
+        return out
+
+    _say("segmented top-t nf=%d t=%d tied=%s" % (nf, t, tied))
+    base = _with_env({"MDK_INV_VARIANT": "0"}, run)
+    for v in "23":
+        other = _with_env({"MDK_INV_VARIANT": v}, run)
+        assert np.array_equal(base.view(np.uint8), other.view(np.uint8)), v
+    if tied:
+        for q in range(len(rc)):
+            assert list(base[q * t]["index"][1:]) == [0, 0, 0]
+
+
 def test_autotune_keeps_results(gpu, gcq):
     rec, lig, _ = gcq
     ng = 48
@@ -330,7 +381,7 @@ def test_autotune_keeps_results(gpu, gcq):
     before = ctx.results(0, 256)
     xv, iv = ctx.autotune(0, 128)
     _say("autotune chose correlation variant %d, inverse variant %d" % (xv, iv))
-    assert 0 <= xv <= 5 and 0 <= iv <= 1
+    assert 3 <= xv <= 5 and 0 <= iv <= 3
     ctx.dock_range(0, 256)
     ctx.sync()
     assert np.array_equal(before.view(np.uint8), ctx.results(0, 256).view(np.uint8))
