@@ -437,7 +437,7 @@ struct mdk_ctx {
     float2 *d_tw = nullptr;
     float2 *d_tw4 = nullptr;  // exp(-2 pi i m / nf): twiddles of the specialised pipeline
     bool use_pipe = false;    // compile-time specialised kernels exist for nf
-    int xcorr_variant = 5;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
+    int xcorr_variant = 6;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
     int cu_count = 256;       // compute units (persistent grids)
     int inv_variant = 2;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
     int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, 1 separate passes
@@ -697,7 +697,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         const char *g = std::getenv("MDK_FORCE_GENERIC");
         c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
-        c->xcorr_variant = xv ? std::atoi(xv) : 5;
+        c->xcorr_variant = xv ? std::atoi(xv) : 6;
         const char *iv = std::getenv("MDK_INV_VARIANT");
         c->inv_variant = iv ? std::atoi(iv) : 2;
         const char *fv = std::getenv("MDK_FWD_VARIANT");
@@ -1019,7 +1019,7 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
         // fastest one can be chosen freely; each is timed twice after one warm-up pass
         float best = 0.f;
         int bx = c->xcorr_variant, bi = c->inv_variant;
-        for (int xv = 3; xv <= 5; xv++)  // 0..2 are far slower on every size measured
+        for (int xv = 3; xv <= 6; xv++)  // 0..2 are far slower on every size measured
             for (int iv = 0; iv <= 3; iv++) {
                 c->xcorr_variant = xv;
                 c->inv_variant = iv;

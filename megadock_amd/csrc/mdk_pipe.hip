@@ -730,7 +730,8 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
     }
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
     // the grid, so consecutive workgroups share the receptor tile in L2, folded lines kept
-    // in the exchange buffer's LDS, 16 columns per workgroup (5, the default) or 8 (4); 8
+    // in the exchange buffer's LDS, 16 columns per workgroup with the receptor values
+    // loaded first (6, the default) or where the fold needs them (5), 8 columns (4); 8
     // columns with a separate fold buffer
     // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
     // of 4 rotations (1); one rotation per workgroup, tile index fastest (2)
@@ -753,8 +754,13 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
             kp_x_corr<N, LX, 1, false, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
                                                                                               a.H, a.Hp, B);
             break;
-        default:  // 5: as 4 with 16 columns per workgroup
+        case 5:  // as 4 with 16 columns per workgroup
             kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+            break;
+        default:  // 6: as 5 with the receptor tile loaded into registers before the forward x-FFT,
+                  // so that its latency hides behind the transform
+            kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
                 a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
     }
     if (a.ev) (void)hipEventRecord(a.ev[2], s);

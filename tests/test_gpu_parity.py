@@ -281,7 +281,7 @@ def test_topn_device_merge_all_tied(gpu, gcq):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "012345"] +
+@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "0123456"] +
                          [{"MDK_INV_VARIANT": v} for v in "123"] + [{"MDK_STREAMS": "2"}, {"MDK_FWD_VARIANT": "1"}])
 def test_tuning_variants_give_identical_results(gpu, gcq, env):
     """Every kernel variant computes the same operations in the same order: results are
@@ -381,7 +381,7 @@ def test_autotune_keeps_results(gpu, gcq):
     before = ctx.results(0, 256)
     xv, iv = ctx.autotune(0, 128)
     _say("autotune chose correlation variant %d, inverse variant %d" % (xv, iv))
-    assert 3 <= xv <= 5 and 0 <= iv <= 3
+    assert 3 <= xv <= 6 and 0 <= iv <= 3
     ctx.dock_range(0, 256)
     ctx.sync()
     assert np.array_equal(before.view(np.uint8), ctx.results(0, 256).view(np.uint8))
