@@ -404,80 +404,94 @@ __device__ __forceinline__ void block_top(TopR<TT> &tl, int t, Cand *wl, Cand *d
 }
 
 // ------------------------------------------------------------------ inverse z + top-t
-// block (row chunk, b): rows y0..y0+L-1 of rotation b, row r = x*N + y; the row's
-// spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps the top-TT
-// (TT >= t, compile time) of its N2 outputs; lists are merged across each wavefront with
-// lane shuffles and across wavefronts through LDS, leaving the block's top-t as t
-// candidates.  The test hook writes the real score rows to smap instead.
+// block (x, b): the N rows r = x*N + y of plane x of rotation b, L rows at a time; the
+// row's spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps a
+// running top-TT (TT >= t, compile time) of its outputs over the plane's row groups; the
+// lists are merged across each wavefront with lane shuffles and across wavefronts through
+// LDS once per plane, leaving the plane's top-t as t candidates.  The test hook writes the
+// real score rows to smap instead.
 template <int N, int L, int TT>
 __global__ void __launch_bounds__(L *Geo<N>::T)
     kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                  const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    constexpr int NROW = N * N, NW = (NT + 63) / 64;
+    constexpr int NROW = N * N, NW = (NT + 63) / 64, RG = N / L;
     static_assert(NT % 64 == 0, "the wavefront top-t needs whole wavefronts");
+    static_assert(N % L == 0, "whole row groups per plane");
     __shared__ float2 S[N * LS];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int b = blockIdx.y, row0 = blockIdx.x * L;
+    const int b = blockIdx.y;
     const float2 *src = cc + ((size_t)b * NROW) * Hp;
-    for (int q = tid; q < L * H; q += NT) {
-        const int l = q / H, kz = q - l * H;
-        float2 v = make_float2(0.f, 0.f);
-        if (row0 + l < NROW) v = src[(size_t)(row0 + l) * Hp + kz];
-        S[kz * LS + l] = v;
-        if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
-    }
-    __syncthreads();
-    {
-        float2 v[N1];
-        if (u < N2)
-            sfor<0, N1>([&](auto jc) {
-                constexpr int j1 = decltype(jc)::value;
-                v[j1] = S[(u + N2 * j1) * LS + line];
-            });
-        __syncthreads();  // step 1 writes back into S
-        if (u < N2) rf::step1_store<N, true>(v, S, tw, LS, line, u);
-    }
-    __syncthreads();
-    const int r = row0 + line;
-    const bool act = u < N1 && r < NROW;
-    float2 y[N2];
-    if (u < N1) rf::step2_load<N, true>(y, S, LS, line, u);
-    if (smap) {  // uniform
+    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
+    // >= tau (the t largest lane maxima of a row group, the largest such bound over the
+    // plane so far, taken from its first four row groups and every fourth after; the t
+    // scores behind it were pushed when it was raised), so anything whose key falls below
+    // tau's loses.  Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1
+    // exactly when t > 1).
+    constexpr bool QUOT = TT > 1;
+    TopR<TT> tl;
+    tl.init();
+    float run_tau = -INFINITY, cut = -INFINITY;
+    for (int g = 0; g < RG; g++) {
+        const int row0 = (blockIdx.x * RG + g) * L;
+        __syncthreads();  // S is free: the previous row group has read it
+        for (int q = tid; q < L * H; q += NT) {
+            const int l = q / H, kz = q - l * H;
+            const float2 v = src[(size_t)(row0 + l) * Hp + kz];
+            S[kz * LS + l] = v;
+            if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
+        }
+        __syncthreads();
+        {
+            float2 v[N1];
+            if (u < N2)
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    v[j1] = S[(u + N2 * j1) * LS + line];
+                });
+            __syncthreads();  // step 1 writes back into S
+            if (u < N2) rf::step1_store<N, true>(v, S, tw, LS, line, u);
+        }
+        __syncthreads();
+        const int r = row0 + line;
+        const bool act = u < N1;
+        float2 y[N2];
+        if (u < N1) rf::step2_load<N, true>(y, S, LS, line, u);
+        if (smap) {  // uniform
+            if (act)
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
+                });
+            continue;
+        }
+        if (g < 4 || (g & 3) == 0) {  // uniform
+            float mx = -INFINITY;
+            if (act)
+                sfor<0, N2>([&](auto kc) { mx = fmaxf(mx, y[decltype(kc)::value].x); });
+            const float tau = wave_kth_max(mx, t);
+            if (tau > run_tau) {  // uniform
+                run_tau = tau;
+                cut = QUOT ? quotient_cut(tau, nf3f) : tau;
+            }
+        }
         if (act)
             sfor<0, N2>([&](auto kc) {
                 constexpr int k2 = decltype(kc)::value;
-                smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
+                const float v = y[k2].x;
+                if (v >= cut) tl.push(QUOT ? v / nf3f : v, r * N + u + N1 * k2);
             });
-        return;
     }
-    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
-    // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
-    // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
-    constexpr bool QUOT = TT > 1;
-    float mx = -INFINITY;
-    if (act)
-        sfor<0, N2>([&](auto kc) { mx = fmaxf(mx, y[decltype(kc)::value].x); });
-    const float tau = wave_kth_max(mx, t);
-    const float cut = QUOT ? quotient_cut(tau, nf3f) : tau;
-    TopR<TT> tl;
-    tl.init();
-    if (act)
-        sfor<0, N2>([&](auto kc) {
-            constexpr int k2 = decltype(kc)::value;
-            const float v = y[k2].x;
-            if (v >= cut) tl.push(QUOT ? v / nf3f : v, r * N + u + N1 * k2);
-        });
+    if (smap) return;  // uniform
     block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
 template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
     constexpr int T = Geo<N>::T;
-    kp_inv_z_top<N, L, TT><<<dim3((N * N + L - 1) / L, a.B), L * T, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H,
-                                                                           a.Hp, a.t, a.nf3f);
+    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), L * T, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, a.t, a.nf3f);
 }
 
 // the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS
@@ -788,7 +802,7 @@ template <int N> hipError_t run_pipe(const PipeArgs &a) {
 
 int pipe_cands_per_rotation(int nf) {
     if (!pipe_available(nf)) return 0;
-    return fused_inverse(nf) ? nf : (nf * nf + 15) / 16;
+    return nf;  // one list per (b, x) plane
 }
 
 bool pipe_available(int nf) {
