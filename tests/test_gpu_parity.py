@@ -281,13 +281,14 @@ def test_topn_device_merge_all_tied(gpu, gcq):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", [{"MDK_XCORR_VARIANT": v} for v in "0123456"] +
-                         [{"MDK_INV_VARIANT": v} for v in "123"] + [{"MDK_STREAMS": "2"}, {"MDK_FWD_VARIANT": "1"}])
-def test_tuning_variants_give_identical_results(gpu, gcq, env):
+@pytest.mark.parametrize("ng,env", [(48, {"MDK_XCORR_VARIANT": v}) for v in "0123456"] +
+                         [(48, {"MDK_INV_VARIANT": v}) for v in "123"] +
+                         [(48, {"MDK_STREAMS": "2"}), (48, {"MDK_FWD_VARIANT": "1"})] +
+                         [(72, {"MDK_XCORR_VARIANT": v}) for v in "456"] + [(72, {"MDK_FWD_VARIANT": "1"})])
+def test_tuning_variants_give_identical_results(gpu, gcq, ng, env):
     """Every kernel variant computes the same operations in the same order: results are
-    bitwise equal to the default's."""
+    bitwise equal to the default's (nf = 96 and 144)."""
     rec, lig, _ = gcq
-    ng = 48
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     ang = O.load_rotations(3600)[:300]
     rc = M.rotation_coefs(ang)
