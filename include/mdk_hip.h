@@ -69,6 +69,13 @@ int mdk_get_receptor_spectrum(mdk_ctx *ctx, float *re, float *im);
 int mdk_set_ligand(mdk_ctx *ctx, const float *xyz, const float *center, const float *radius,
                    const float *charge, const float *grid_coord, float gw, float core_delta);
 
+/* As mdk_set_ligand for a ligand of na atoms, 1 <= na <= the na_lig the context was
+ * created with, so that one context serves the pairs of a job table that share an FFT
+ * size.  No counterpart in the reference, which builds its FFTProcess per pair (the
+ * table farm of src/mpidp.cpp runs one process per job). */
+int mdk_set_ligand_atoms(mdk_ctx *ctx, int na, const float *xyz, const float *center, const float *radius,
+                         const float *charge, const float *grid_coord, float gw, float core_delta);
+
 /* Dock B rotations given by host-computed coefficients rotcoef (B*9, row-major
  * r11..r33 as src/docking.cpp:376-384 computes them in float) with rotation numbers
  * rot_id (B); writes B*t results (the _Top entries of src/fft_process.cpp:640-643).

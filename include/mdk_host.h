@@ -42,6 +42,13 @@ int mdkh_receptor_voxel(const mdkh_protein *rec, int ng, float gw, float acerati
  * reads the PDBs, sizes the grids, docks on GPU 0 and writes the .out file. */
 int mdkh_run(int argc, const char **argv);
 
+/* The job-table farm with the reference's table format and options (src/mpidp.cpp:
+ * -tb table, -lg log, -rt retries, -ot output field; TITLE= / PARAM= lines, $1..$n
+ * replaced by each job line's tab-separated fields), run in one process: host threads
+ * prepare jobs (-pj threads) while GPU 0 docks them.  -dry writes the expanded command
+ * lines to the log and runs nothing.  Nonzero when any job failed after its retries. */
+int mdkh_run_table(int argc, const char **argv);
+
 const char *mdkh_last_error(void);
 
 #ifdef __cplusplus

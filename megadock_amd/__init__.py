@@ -22,12 +22,12 @@ ABI_SYMBOLS = [
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
-    "mdk_autotune",
+    "mdk_autotune", "mdk_set_ligand_atoms",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
     "mdkh_protein_load", "mdkh_protein_free", "mdkh_protein_size", "mdkh_protein_atoms",
-    "mdkh_autogrid", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_last_error",
+    "mdkh_autogrid", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_run_table", "mdkh_last_error",
 ]
 STAGES = ["voxelize", "fwd_z", "fwd_y", "x_corr", "inv_y", "inv_z_top", "reduce"]
 
@@ -122,11 +122,17 @@ class Context:
         return re, im
 
     def set_ligand(self, xyz, center, radius, charge, grid_coord, gw, delta=1.0):
+        """Ligand atoms; fewer than the context's na go through mdk_set_ligand_atoms."""
         xyz = np.ascontiguousarray(xyz, np.float32)
-        assert xyz.size == 3 * self.na
+        na = xyz.size // 3
+        assert xyz.size == 3 * na and 1 <= na <= self.na
         args = [np.ascontiguousarray(a, np.float32) for a in (center, radius, charge, grid_coord)]
-        _check(lib().mdk_set_ligand(self.h, _fp(xyz), *[_fp(a) for a in args], C.c_float(gw), C.c_float(delta)),
-               "mdk_set_ligand")
+        if na == self.na:
+            _check(lib().mdk_set_ligand(self.h, _fp(xyz), *[_fp(a) for a in args], C.c_float(gw),
+                                        C.c_float(delta)), "mdk_set_ligand")
+        else:
+            _check(lib().mdk_set_ligand_atoms(self.h, C.c_int(na), _fp(xyz), *[_fp(a) for a in args],
+                                              C.c_float(gw), C.c_float(delta)), "mdk_set_ligand_atoms")
 
     def dock(self, rotcoef, rot_id):
         rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
@@ -291,3 +297,11 @@ def run(args):
     argv = ["megadock-hip"] + list(args)
     arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
     _hcheck(lib().mdkh_run(C.c_int(len(argv)), arr), "mdkh_run")
+
+
+def run_table(args):
+    """The job-table farm (megadock-hip-dp), e.g. ["-tb", table, "-lg", log]: every job
+    line of the table expands the table's PARAM line into a megadock command line."""
+    argv = ["megadock-hip-dp"] + list(args)
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    _hcheck(lib().mdkh_run_table(C.c_int(len(argv)), arr), "mdkh_run_table")

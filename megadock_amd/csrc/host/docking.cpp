@@ -25,7 +25,10 @@ double now() {
 
 std::string abi_error(const char *what) { return std::string(what) + ": " + mdk_last_error(); }
 
-// rotations per device batch: about 2 GiB of per-batch work buffers, at most 128
+const float PI_F = (float)(4.0 * std::atan(1.0));  // src/constant.h:24
+
+}  // namespace
+
 int auto_batch(int nf, int ng, int t) {
     const double H = nf / 2 + 1;
     const double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
@@ -33,10 +36,6 @@ int auto_batch(int nf, int ng, int t) {
     int b = (int)(8.0 * 1024 * 1024 * 1024 / per_rot);  // 8 GiB of batch buffers
     return std::max(1, std::min(128, b));
 }
-
-const float PI_F = (float)(4.0 * std::atan(1.0));  // src/constant.h:24
-
-}  // namespace
 
 // ------------------------------------------------------------------ FFTProcess
 FFTProcess::~FFTProcess() {
@@ -150,28 +149,33 @@ int Docking::dockz(std::string &err) {
     return 0;
 }
 
-int Docking::output(std::string &err) {
-    FILE *f = std::fopen(prm_->out_file.c_str(), "w");
+int write_decoys(const Parameter &prm, const Protein &rec, const Protein &lig, int num_grid, const SortScore *top,
+                 std::string &err) {
+    FILE *f = std::fopen(prm.out_file.c_str(), "w");
     if (!f) {
-        err = "[ERROR] Output file [" + prm_->out_file + "] not open!!";
+        err = "[ERROR] Output file [" + prm.out_file + "] not open!!";
         return 1;
     }
-    std::fprintf(f, "%d\t%.2f\n", 2 * num_grid_, prm_->grid_width);  // src/docking.cpp:413-414
-    std::fprintf(f, "\t%f\t%f\t%f\n", 0.0, 0.0, 0.0);                 // receptor angle
-    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm_->rec_file.c_str(), rec_->center(0), rec_->center(1), rec_->center(2));
-    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm_->lig_file.c_str(), lig_->center(0), lig_->center(1), lig_->center(2));
-    for (int rank = 0; rank < prm_->num_output; rank++) {  // src/docking.cpp:434-448
-        const int n = fft_->top_index(rank, 0);
+    std::fprintf(f, "%d\t%.2f\n", 2 * num_grid, prm.grid_width);  // src/docking.cpp:413-414
+    std::fprintf(f, "\t%f\t%f\t%f\n", 0.0, 0.0, 0.0);             // receptor angle
+    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm.rec_file.c_str(), rec.center(0), rec.center(1), rec.center(2));
+    std::fprintf(f, "%s\t%f\t%f\t%f\n", prm.lig_file.c_str(), lig.center(0), lig.center(1), lig.center(2));
+    for (int rank = 0; rank < prm.num_output; rank++) {  // src/docking.cpp:434-448
+        const int n = top[rank].index[0];
         float th[3];
         for (int k = 0; k < 3; k++) {
-            th[k] = prm_->zangle[3 * n + k];
+            th[k] = prm.zangle[3 * n + k];
             if (th[k] >= PI_F) th[k] = (float)(th[k] - PI_F * 2.0);
         }
-        std::fprintf(f, "%f\t%f\t%f\t%d\t%d\t%d\t%.2f\n", th[0], th[1], th[2], fft_->top_index(rank, 1),
-                     fft_->top_index(rank, 2), fft_->top_index(rank, 3), fft_->top_score(rank));
+        std::fprintf(f, "%f\t%f\t%f\t%d\t%d\t%d\t%.2f\n", th[0], th[1], th[2], top[rank].index[1],
+                     top[rank].index[2], top[rank].index[3], top[rank].score);
     }
     std::fclose(f);
     return 0;
+}
+
+int Docking::output(std::string &err) {
+    return write_decoys(*prm_, *rec_, *lig_, num_grid_, fft_->top(), err);
 }
 
 // ------------------------------------------------------------------ run

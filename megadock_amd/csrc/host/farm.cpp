@@ -1,0 +1,523 @@
+// Job-table farm: the in-process counterpart of the reference's MPI master/worker farm
+// (src/mpidp.cpp).  The table format and option names are the reference's:
+//   TITLE=<title>            (src/mpidp.cpp:212-217)
+//   PARAM=<command line>     with $1, $2, ... replaced by the fields of a job line
+//                            (src/mpidp.cpp:218-224, :787-834, replace_pattern :877-897)
+//   <field 1>\t<field 2>...  one job per line
+// and -tb table, -lg log, -rt retries, -ot k (field k names the output file; a retry
+// appends ".<try>" to it, src/mpidp.cpp:803-810), -wl / -pg (accepted, no effect here);
+// every other option is passed to each job in front of the expanded PARAM line.
+//
+// Instead of one process per job over MPI ranks, one process keeps the GPU busy:
+//   prep threads (-pj, default min(8, cores/2)) do each job's CPU work: command line,
+//     PDB reading, grid sizing, the receptor's shape/electrostatics grids (cached per
+//     receptor file and grid, since a table repeats receptors), rotation coefficients;
+//   the calling thread docks the prepared jobs in turn on GPU 0 through libmdk contexts
+//     kept per FFT size (re-created only when a job needs more ligand atoms, another t or
+//     another batch) and writes each job's .out file.
+// A failed job is put back in the queue until it has run 1 + retries times
+// (src/mpidp.cpp:372-543); the log ends with the per-job table (src/mpidp.cpp:900-935).
+#include <sys/time.h>
+
+#include <algorithm>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "mdk_host.h"
+
+namespace mdk {
+namespace {
+
+double now_s() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+struct FarmOptions {
+    std::string table, log = "./master.log";
+    int out_option = 0, retries = 0, prep_threads = 0;
+    bool dry = false;
+    std::vector<std::string> app;  // options passed to every job
+};
+
+// src/mpidp.cpp:148-191: farm options; the rest belongs to the application
+int parse_options(int argc, char **argv, FarmOptions &o, std::string &err) {
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        auto value = [&](std::string &v) {
+            if (i + 1 >= argc) {
+                err = "option " + a + " needs a value";
+                return false;
+            }
+            v = argv[++i];
+            return true;
+        };
+        std::string v;
+        if (a == "-tb") {
+            if (!value(o.table)) return 1;
+        } else if (a == "-lg") {
+            if (!value(o.log)) return 1;
+        } else if (a == "-ot") {
+            if (!value(v)) return 1;
+            o.out_option = std::atoi(v.c_str());
+        } else if (a == "-rt") {
+            if (!value(v)) return 1;
+            o.retries = std::max(0, std::atoi(v.c_str()));
+        } else if (a == "-wl" || a == "-pg") {
+            if (!value(v)) return 1;
+        } else if (a == "-pj") {
+            if (!value(v)) return 1;
+            o.prep_threads = std::atoi(v.c_str());
+        } else if (a == "-dry") {
+            o.dry = true;
+        } else {
+            o.app.push_back(a);
+        }
+    }
+    if (o.table.empty()) {
+        err = "[ERROR] no job table: -tb <table file>";
+        return 1;
+    }
+    return 0;
+}
+
+// src/mpidp.cpp:251-275: spaces among the first ip characters are deleted
+std::string erase_space(std::string s, size_t ip) {
+    for (;;) {
+        const size_t n = s.find(' ');
+        if (n == std::string::npos || n >= ip) return s;
+        s.erase(n, 1);
+    }
+}
+
+// src/mpidp.cpp:877-897
+std::string replace_pattern(const std::string &pattern, const std::string &position, const std::string &option) {
+    std::string out;
+    size_t before = 0, pos = 0;
+    while ((pos = pattern.find(position, pos)) != std::string::npos) {
+        out.append(pattern, before, pos - before);
+        out.append(option);
+        pos += position.size();
+        before = pos;
+    }
+    out.append(pattern, before, std::string::npos);
+    return out;
+}
+
+struct Table {
+    std::string title = "MasterProc", param = "MPIDP";
+    std::vector<std::string> jobs;
+};
+
+// src/mpidp.cpp:198-246 (an empty line is not a job here)
+int read_table(const std::string &path, Table &t, std::string &err) {
+    std::ifstream in(path);
+    if (!in) {
+        err = "[ERROR] Table file [" + path + "] was not opened!!";
+        return 1;
+    }
+    std::string line;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        line = erase_space(line, 7);
+        if (!strncmp(line.c_str(), "TITLE=", 6) || !strncmp(line.c_str(), "Title=", 6) ||
+            !strncmp(line.c_str(), "title=", 6))
+            t.title = line.substr(6);
+        else if (!strncmp(line.c_str(), "PARAM=", 6) || !strncmp(line.c_str(), "Param=", 6) ||
+                 !strncmp(line.c_str(), "param=", 6))
+            t.param = line.substr(6);
+        else if (!line.empty())
+            t.jobs.push_back(line);
+    }
+    return 0;
+}
+
+std::vector<std::string> split(const std::string &s, char sep) {
+    std::vector<std::string> out;
+    std::string cur;
+    for (char ch : s) {
+        if (ch == sep) {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur += ch;
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+// The job's command line: the farm's other options, then PARAM with $i replaced by
+// field i (src/mpidp.cpp:796-822; tokens split at spaces, empty tokens dropped).
+std::vector<std::string> expand_job(const FarmOptions &o, const Table &t, const std::string &line, int attempt) {
+    std::vector<std::string> fields = split(line, '\t');
+    std::string p = t.param;
+    for (size_t i = 0; i < fields.size(); i++) {
+        std::string f = fields[i];
+        if (o.out_option == (int)i + 1 && attempt > 0) f += "." + std::to_string(attempt);
+        p = replace_pattern(p, "$" + std::to_string(i + 1), f);
+    }
+    std::vector<std::string> argv = {"megadock-hip"};
+    argv.insert(argv.end(), o.app.begin(), o.app.end());
+    for (const auto &tok : split(p, ' '))
+        if (!tok.empty()) argv.push_back(tok);
+    return argv;
+}
+
+// ------------------------------------------------------------------ prepared jobs
+struct Grids {
+    std::vector<float> r, i;
+};
+
+// receptor grids shared by the jobs that dock the same receptor on the same grid
+class GridCache {
+public:
+    using Key = std::tuple<std::string, int, float, float, float, float>;
+    std::shared_ptr<const Grids> get(const Key &k, const Receptor &rec, int ng, const Parameter &p) {
+        std::shared_ptr<Entry> e;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &slot = map_[k];
+            if (!slot) slot = std::make_shared<Entry>();
+            e = slot;
+        }
+        std::call_once(e->once, [&] {
+            auto gr = std::make_shared<Grids>();
+            const size_t ng3 = (size_t)ng * ng * ng;
+            gr->r.resize(ng3);
+            gr->i.resize(ng3);
+            std::vector<float> xyz(3 * (size_t)rec.num_atoms());
+            rec.shift_center(xyz.data());
+            rec.voxelize(ng, p.grid_width, grid_coord(ng, p.grid_width), xyz.data(), p.ace_ratio, p.rec_core,
+                         p.elec_ratio, gr->r.data(), gr->i.data());
+            e->grids = gr;
+        });
+        return e->grids;
+    }
+
+private:
+    struct Entry {
+        std::once_flag once;
+        std::shared_ptr<const Grids> grids;
+    };
+    std::mutex mu_;
+    std::map<Key, std::shared_ptr<Entry>> map_;
+};
+
+struct Job {
+    int id = 0;          // 1-based event number (src/mpidp.cpp:313)
+    int attempt = 0;     // 0 for the first run, then 1..retries
+    std::string line;
+    // filled by preparation
+    std::vector<std::string> argv;
+    Parameter prm;
+    std::unique_ptr<Receptor> rec;
+    std::unique_ptr<Ligand> lig;
+    int ng = 0;
+    std::vector<float> gc, rotcoef;
+    std::shared_ptr<const Grids> grids;
+    std::string err;
+    double t_prep = 0, t_dock = 0;
+};
+
+struct JobRecord {
+    std::string line, status = "not run", out_file, message;
+    int tries = 0, ng = 0, rotations = 0;
+    double seconds = 0;
+};
+
+std::mutex g_getopt_mu;  // Parameter::initialize uses getopt's global state
+
+int prepare(Job &j, GridCache &cache) {
+    const double t0 = now_s();
+    {
+        std::vector<char *> av;
+        for (auto &a : j.argv) av.push_back(const_cast<char *>(a.c_str()));
+        av.push_back(nullptr);
+        std::lock_guard<std::mutex> g(g_getopt_mu);
+        if (j.prm.initialize((int)j.argv.size(), av.data(), j.err)) return 1;
+    }
+    Parameter &p = j.prm;
+    if (p.gpu_limit == 0) {
+        j.err = "-G 0 asks for the CPU loop, which this build does not provide";
+        return 1;
+    }
+    j.rec.reset(new Receptor(p.rec_file));
+    j.lig.reset(new Ligand(p.lig_file));
+    if (j.rec->initialize(p, j.err) || j.lig->initialize(p, j.err)) return 1;
+    if (!p.num_fft_flag) {
+        const int a = autogrid_receptor(*j.rec, p), b = autogrid_ligand(*j.lig, p);
+        if (a < 0 || b < 0) {
+            j.err = "grid sizing failed (protein larger than the FFT size table)";
+            return 1;
+        }
+        j.ng = std::max(a, b);
+    } else {
+        if (check_grid(*j.rec, *j.lig, p, j.err)) return 1;
+        j.ng = p.num_grid;
+    }
+    j.gc = grid_coord(j.ng, p.grid_width);
+    j.grids = cache.get(GridCache::Key{p.rec_file, j.ng, p.grid_width, p.ace_ratio, p.rec_core, p.elec_ratio},
+                        *j.rec, j.ng, p);
+    j.rotcoef.resize(9 * (size_t)p.num_rot_angles);
+    if (mdk_rotation_coef(p.zangle.data(), p.num_rot_angles, j.rotcoef.data())) {
+        j.err = std::string("mdk_rotation_coef: ") + mdk_last_error();
+        return 1;
+    }
+    j.t_prep = now_s() - t0;
+    return 0;
+}
+
+// ------------------------------------------------------------------ device side
+struct Slot {
+    mdk_ctx *ctx = nullptr;
+    int na = 0, t = 0, batch = 0;
+    int rot_set = -1;  // rotation set loaded into the context
+};
+
+class Contexts {
+public:
+    ~Contexts() {
+        for (auto &kv : slots_)
+            if (kv.second.ctx) mdk_destroy(kv.second.ctx);
+    }
+    // context for the job's FFT size, able to hold its ligand, t and batch
+    mdk_ctx *get(const Job &j, std::string &err) {
+        const int nf = 2 * j.ng, na = j.lig->num_atoms(), t = j.prm.num_sort;
+        int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
+        batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
+        Slot &s = slots_[nf];
+        if (s.ctx && (s.na < na || s.t != t || s.batch < batch)) {
+            mdk_destroy(s.ctx);
+            s = Slot();
+        }
+        if (!s.ctx) {
+            if (mdk_create(0, nf, j.ng, na, t, batch, &s.ctx)) {
+                err = std::string("mdk_create: ") + mdk_last_error();
+                s = Slot();
+                return nullptr;
+            }
+            s.na = na;
+            s.t = t;
+            s.batch = batch;
+        }
+        return s.ctx;
+    }
+    int &rot_set(int nf) { return slots_[nf].rot_set; }
+
+private:
+    std::map<int, Slot> slots_;
+};
+
+int dock(Job &j, Contexts &cs) {
+    const double t0 = now_s();
+    Parameter &p = j.prm;
+    mdk_ctx *ctx = cs.get(j, j.err);
+    if (!ctx) return 1;
+    auto abi = [&](const char *what) {
+        j.err = std::string(what) + ": " + mdk_last_error();
+        return 1;
+    };
+    int &loaded = cs.rot_set(2 * j.ng);
+    if (loaded != p.rotation_set) {
+        if (mdk_set_rotations(ctx, j.rotcoef.data(), p.num_rot_angles)) return abi("mdk_set_rotations");
+        loaded = p.rotation_set;
+    }
+    if (mdk_set_receptor(ctx, j.grids->r.data(), j.grids->i.data())) return abi("mdk_set_receptor");
+    const Ligand &L = *j.lig;
+    if (mdk_set_ligand_atoms(ctx, L.num_atoms(), L.coords().data(), L.center(), L.radius.data(), L.charge.data(),
+                             j.gc.data(), p.grid_width, p.lig_core))
+        return abi("mdk_set_ligand_atoms");
+    if (mdk_dock_range(ctx, 0, p.num_rot_angles)) return abi("mdk_dock_range");
+    std::vector<mdk_sortscore> best(std::max(1, p.num_output));
+    if (mdk_topn(ctx, p.num_rot_angles, p.num_output, best.data())) return abi("mdk_topn");
+    std::vector<SortScore> top(best.size());
+    std::memcpy(top.data(), best.data(), sizeof(SortScore) * best.size());
+    if (write_decoys(p, *j.rec, L, j.ng, top.data(), j.err)) return 1;
+    j.t_dock = now_s() - t0;
+    return 0;
+}
+
+template <class T> class Queue {
+public:
+    void push(T v) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            q_.push_back(std::move(v));
+        }
+        cv_.notify_one();
+    }
+    // blocks until an item or close(); false when closed and empty
+    bool pop(T &v) {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return !q_.empty() || closed_; });
+        if (q_.empty()) return false;
+        v = std::move(q_.front());
+        q_.pop_front();
+        return true;
+    }
+    void close() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            closed_ = true;
+        }
+        cv_.notify_all();
+    }
+
+private:
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<T> q_;
+    bool closed_ = false;
+};
+
+}  // namespace
+
+int run_table(int argc, char **argv, std::string &err) {
+    const double t_start = now_s();
+    FarmOptions o;
+    if (parse_options(argc, argv, o, err)) return 1;
+    Table t;
+    if (read_table(o.table, t, err)) return 1;
+    std::ofstream log(o.log);
+    if (!log) {
+        err = "[ERROR] Log file [" + o.log + "] was not opened!!";
+        return 1;
+    }
+    log << " MEGADOCK-compatible job farm on MI355X (libmdk)\n\n";
+    log << "Table file    : -tb " << o.table << "\n";
+    if (o.out_option) log << "Output option : -ot " << o.out_option << "\n";
+    if (o.retries) log << "Retry option  : -rt " << o.retries << "\n";
+    if (!o.app.empty()) {
+        log << "Other options :";
+        for (auto &a : o.app) log << " " << a;
+        log << "\n";
+    }
+    log << "\nTITLE=" << t.title << "\nPARAM=" << t.param << "\n\n";
+    const int njobs = (int)t.jobs.size();
+    std::vector<JobRecord> rec(njobs);
+    for (int i = 0; i < njobs; i++) rec[i].line = t.jobs[i];
+    if (o.dry) {  // the command line of every job, nothing run
+        for (int i = 0; i < njobs; i++) {
+            char name[16];
+            std::snprintf(name, sizeof name, "%05d", i + 1);
+            log << name << "\t";
+            const auto av = expand_job(o, t, t.jobs[i], 0);
+            for (size_t k = 0; k < av.size(); k++) log << (k ? " " : "") << av[k];
+            log << "\n";
+        }
+        return 0;
+    }
+    int ndev = 0;
+    if (mdk_device_count(&ndev) || ndev < 1) {
+        err = "no GPU visible: this build runs the rotation loop on the GPU";
+        return 1;
+    }
+    int nprep = o.prep_threads;
+    if (nprep < 1) nprep = std::max(1, std::min(8, (int)std::thread::hardware_concurrency() / 2));
+
+    Queue<std::unique_ptr<Job>> todo, ready;
+    std::mutex rec_mu;
+    int outstanding = njobs;  // jobs neither finished nor given up (guarded by rec_mu)
+    for (int i = 0; i < njobs; i++) {
+        std::unique_ptr<Job> j(new Job());
+        j->id = i + 1;
+        j->line = t.jobs[i];
+        todo.push(std::move(j));
+    }
+    if (njobs == 0) todo.close();
+    // a job that failed: back into the queue while it has tries left
+    auto finish = [&](std::unique_ptr<Job> j, bool ok) {
+        std::lock_guard<std::mutex> g(rec_mu);
+        JobRecord &r = rec[j->id - 1];
+        r.tries = j->attempt + 1;
+        r.ng = j->ng;
+        r.rotations = j->prm.num_rot_angles;
+        r.seconds += j->t_prep + j->t_dock;
+        r.out_file = j->prm.out_file;
+        if (ok) {
+            r.status = "done";
+            r.message.clear();
+        } else {
+            r.status = "failed";
+            r.message = j->err;
+            if (j->attempt < o.retries) {
+                std::unique_ptr<Job> again(new Job());
+                again->id = j->id;
+                again->line = j->line;
+                again->attempt = j->attempt + 1;
+                todo.push(std::move(again));
+                return;
+            }
+        }
+        if (--outstanding == 0) todo.close();
+    };
+    GridCache cache;
+    std::vector<std::thread> workers;
+    std::mutex ready_mu;
+    int prep_alive = nprep;
+    for (int w = 0; w < nprep; w++)
+        workers.emplace_back([&] {
+            std::unique_ptr<Job> j;
+            while (todo.pop(j)) {
+                j->argv = expand_job(o, t, j->line, j->attempt);
+                if (prepare(*j, cache)) {
+                    j->t_prep = 0;
+                    finish(std::move(j), false);
+                } else {
+                    ready.push(std::move(j));
+                }
+            }
+            std::lock_guard<std::mutex> g(ready_mu);
+            if (--prep_alive == 0) ready.close();
+        });
+    {
+        Contexts cs;
+        std::unique_ptr<Job> j;
+        while (ready.pop(j)) {
+            const bool ok = dock(*j, cs) == 0;
+            std::printf("# job %05d (%s): %s\n", j->id, j->line.c_str(), ok ? "done" : j->err.c_str());
+            std::fflush(stdout);
+            finish(std::move(j), ok);
+        }
+    }
+    for (auto &w : workers) w.join();
+
+    // per-job report (src/mpidp.cpp:900-935)
+    int failed = 0;
+    long long rotations = 0;
+    log << "JOB\tTRIES\tSTATUS\tNG\tROTATIONS\tSECONDS\tOUTPUT\tLINE\n";
+    for (int i = 0; i < njobs; i++) {
+        const JobRecord &r = rec[i];
+        char name[16];
+        std::snprintf(name, sizeof name, "%05d", i + 1);
+        log << name << "\t" << r.tries << "\t" << r.status << "\t" << r.ng << "\t" << r.rotations << "\t" << r.seconds
+            << "\t" << r.out_file << "\t" << r.line;
+        if (!r.message.empty()) log << "\t# " << r.message;
+        log << "\n";
+        if (r.status != "done") failed++;
+        else rotations += r.rotations;
+    }
+    const double el = now_s() - t_start;
+    log << "\nJobs done = " << njobs - failed << " / " << njobs << "\nElapsed time  = " << el << " sec.\n"
+        << "Rotations docked = " << rotations << " (" << (el > 0 ? rotations / el : 0.0) << " per second)\n";
+    if (failed) {
+        err = std::to_string(failed) + " of " + std::to_string(njobs) + " jobs failed (see " + o.log + ")";
+        return 1;
+    }
+    return 0;
+}
+
+}  // namespace mdk

@@ -111,4 +111,11 @@ int mdkh_run(int argc, const char **argv) {
     return 0;
 }
 
+int mdkh_run_table(int argc, const char **argv) {
+    std::string err;
+    const int rc = mdk::run_table(argc, const_cast<char **>(argv), err);
+    if (rc) return herr(err);
+    return 0;
+}
+
 }  // extern "C"

@@ -136,6 +136,7 @@ public:
     int dock_rotations(int first, int count, std::string &err);
     int sort_index(std::string &err);  // :469-479, merged on the device
     void top_score_clean();            // :482-498
+    const SortScore *top() const { return top_.data(); }
     float top_score(int j) const { return top_[j].score; }
     int top_index(int j, int k) const { return top_[j].index[k]; }
     int num_fft() const { return num_fft_; }
@@ -173,5 +174,15 @@ private:
 // Whole run as the reference's main(): parse flags, read PDBs, size grids, dock, write
 // the .out file (src/main.cpp:38-140, src/control.cpp:18-88, :468-529).
 int run(int argc, char **argv, std::string &err);
+
+// The .out decoy file (src/docking.cpp:399-453) from the merged top list (num_output entries).
+int write_decoys(const Parameter &prm, const Protein &rec, const Protein &lig, int num_grid, const SortScore *top,
+                 std::string &err);
+
+// Rotations per device batch for an FFT size: 8 GiB of batch buffers, at most 128.
+int auto_batch(int nf, int ng, int t);
+
+// The job-table farm (src/mpidp.cpp), in process: see farm.cpp.
+int run_table(int argc, char **argv, std::string &err);
 
 }  // namespace mdk

@@ -793,12 +793,18 @@ int mdk_get_receptor_spectrum(mdk_ctx *c, float *re, float *im) {
 
 int mdk_set_ligand(mdk_ctx *c, const float *xyz, const float *center, const float *radius,
                    const float *charge, const float *grid_coord, float gw, float delta) {
+    if (!c) return fail("mdk_set_ligand: null context");
+    return mdk_set_ligand_atoms(c, c->na, xyz, center, radius, charge, grid_coord, gw, delta);
+}
+
+int mdk_set_ligand_atoms(mdk_ctx *c, int na, const float *xyz, const float *center, const float *radius,
+                         const float *charge, const float *grid_coord, float gw, float delta) {
     if (!c || !xyz || !center || !radius || !charge || !grid_coord) return fail("mdk_set_ligand: null argument");
+    if (na < 1 || na > c->na) return fail("mdk_set_ligand: atom count outside 1..na_lig of the context");
     if (delta == 0.0f) return fail("mdk_set_ligand: core value 0 is not supported (the reference's last shape sweep is order-dependent then)");
     if (!(gw > 0.0f)) return fail("mdk_set_ligand: grid width must be positive");
     HIPCHK(hipSetDevice(c->device));
     drop_graphs(c);  // they hold the ligand's kernel arguments
-    const int na = c->na;
     std::vector<float> rc2(na), rs2(na);
     for (int i = 0; i < na; i++) {  // src/fft_process.cpp:784-787 / src/ligand.cpp:254-257
         rc2[i] = radius[i] * radius[i] * 1.5f;

@@ -1,0 +1,55 @@
+"""GPU: the job-table farm (mdkh_run_table) docks every job of a table exactly as the
+single-pair program does (byte-identical .out files), with one context per FFT size
+reused across jobs (ligands of fewer atoms than the context was made for), and retries
+a failing job before it reports it."""
+import os
+import re
+
+import pytest
+
+import megadock_amd as M
+from megadock_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(log):
+    return {l.split("\t")[0]: l.split("\t") for l in log.read_text().splitlines() if re.match(r"^\d{5}\t", l)}
+
+
+def test_farm_matches_single_runs(tmp_path, golden_dir):
+    r = os.path.join(golden_dir, "1gcq_r.pdb")
+    l = os.path.join(golden_dir, "1gcq_l.pdb")
+    wl = tmp_path / "wl"
+    wl.mkdir()
+    # the reference's sample table (data/SAMPLE.table: every receptor/ligand combination of
+    # 1gcq, which repeats FFT sizes with ligands of different sizes) and two c2 pairs
+    pairs = [(r, r), (r, l), (l, r), (l, l)] + [W.make_c2_pair(a, b, str(wl)) for a, b in [(1, 0), (4, 5)]]
+    lines = ["%s\t%s\t%s" % (a, b, tmp_path / ("farm_%d" % k)) for k, (a, b) in enumerate(pairs)]
+    tb = tmp_path / "jobs.table"
+    tb.write_text("TITLE=farm test\nPARAM=-R $1 -L $2 -o $3.out -N 100 -t 2\n" + "\n".join(lines) + "\n")
+    log = tmp_path / "master.log"
+    print("farm of %d jobs" % len(pairs), flush=True)
+    M.run_table(["-tb", str(tb), "-lg", str(log), "-pj", "4"])
+    rows = _rows(log)
+    assert all(rows["%05d" % (k + 1)][2] == "done" for k in range(len(pairs)))
+    for k, (a, b) in enumerate(pairs):
+        single = tmp_path / ("single_%d.out" % k)
+        M.run(["-R", a, "-L", b, "-o", str(single), "-N", "100", "-t", "2"])
+        assert (tmp_path / ("farm_%d.out" % k)).read_bytes() == single.read_bytes(), (k, a, b)
+
+
+def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):
+    r = os.path.join(golden_dir, "1gcq_r.pdb")
+    l = os.path.join(golden_dir, "1gcq_l.pdb")
+    tb = tmp_path / "jobs.table"
+    tb.write_text("PARAM=-R $1 -L $2 -o $3 -N 10\n%s\t%s\t%s\n%s\t%s\t%s\n"
+                  % (r, l, tmp_path / "ok.out", r, tmp_path / "missing.pdb", tmp_path / "bad.out"))
+    log = tmp_path / "master.log"
+    with pytest.raises(M.MdkError, match="1 of 2 jobs failed"):
+        M.run_table(["-tb", str(tb), "-lg", str(log), "-rt", "2", "-ot", "3"])
+    rows = _rows(log)
+    assert rows["00001"][1:3] == ["1", "done"]
+    assert rows["00002"][1:3] == ["3", "failed"]  # the first run and two retries
+    assert (tmp_path / "ok.out").exists()
+    assert len((tmp_path / "ok.out").read_text().splitlines()) == 4 + 10
