@@ -69,12 +69,57 @@ def rocfft_reference(nf, batch, reps=5):
             "GBps_at_B_rot": round(batch * 32.0 * nf ** 3 / (ms.value * 1e-3) / 1e9, 1)}
 
 
+def bench_c2(args, M, workloads):
+    """c2: a job table of receptor/ligand pairs of uneven sizes (residue counts log-uniform
+    in [50, 800], auto grid, 3600 rotations, t = 1) through the in-process farm
+    (megadock-hip-dp: host threads prepare jobs while the GPU docks), against the same jobs
+    run one after another by the single-pair program."""
+    import tempfile
+    nr, nl = [int(x) for x in args.pairs.split("x")]
+    pairs = [workloads.make_c2_pair(r, l, args.cache) for r in range(nr) for l in range(nl)]
+    work = tempfile.mkdtemp(prefix="mdk_c2_")
+    lines = ["%s\t%s\t%s" % (a, b, os.path.join(work, "farm_%03d.out" % k)) for k, (a, b) in enumerate(pairs)]
+    table = os.path.join(work, "c2.table")
+    with open(table, "w") as f:
+        f.write("TITLE=c2 sample\nPARAM=-R $1 -L $2 -o $3\n" + "\n".join(lines) + "\n")
+    flog = os.path.join(work, "master.log")
+    M.run_table(["-tb", table, "-lg", flog])  # warm-up: runtime, code objects, file cache
+    t0 = time.time()
+    M.run_table(["-tb", table, "-lg", flog])
+    farm_s = time.time() - t0
+    rows = [l.split("\t") for l in open(flog).read().splitlines() if l[:5].isdigit() and "\t" in l]
+    nfs = sorted(2 * int(r[3]) for r in rows)
+    rot = 3600 * len(pairs)
+    log("farm: %d pairs, %d rotations in %.2f s" % (len(pairs), rot, farm_s))
+    t0 = time.time()
+    for k, (a, b) in enumerate(pairs):
+        M.run(["-R", a, "-L", b, "-o", os.path.join(work, "single_%03d.out" % k)])
+    seq_s = time.time() - t0
+    same = all(open(os.path.join(work, "farm_%03d.out" % k), "rb").read()
+               == open(os.path.join(work, "single_%03d.out" % k), "rb").read() for k in range(len(pairs)))
+    log("one after another: %.2f s; outputs identical: %s" % (seq_s, same))
+    spec = set(M.TUNED_NF + M.FURTHER_NF)
+    print(json.dumps({
+        "metric": "c2 job table: rotations/s over receptor/ligand pairs of uneven FFT size",
+        "value": round(rot / farm_s, 1), "unit": "rotations/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c2: first %d receptors x %d ligands of the 32 x 32 table, auto grid, 3600 rotations, t = 1"
+                   % (nr, nl), "pairs": len(pairs), "nf": {str(n): nfs.count(n) for n in sorted(set(nfs))},
+                   "pairs_on_specialised_kernels": sum(1 for n in nfs if n in spec)},
+        "farm_seconds": round(farm_s, 3), "pairs_per_s": round(len(pairs) / farm_s, 2),
+        "one_after_another": {"seconds": round(seq_s, 3), "rotations_per_s": round(rot / seq_s, 1),
+                              "farm_speedup": round(seq_s / farm_s, 2), "outputs_identical": same},
+    }), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--pairs", default="4x8", help="c2: receptors x ligands of the 32 x 32 table to run (first ones)")
     ap.add_argument("--batch", type=int, default=0, help="rotations per device batch (0: automatic)")
     ap.add_argument("--plain-rotations", type=int, default=200)
     ap.add_argument("--cpu-rotations", type=int, default=0, help="CPU baseline sample size (0: automatic)")
@@ -95,6 +140,9 @@ def main():
 
     import megadock_amd as M
     from megadock_amd import workloads
+
+    if args.workload == "c2":
+        return bench_c2(args, M, workloads)
 
     t_setup = time.time()
     rec_pdb, lig_pdb, cfg = workloads.make_pair(args.workload, args.cache)

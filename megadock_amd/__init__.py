@@ -30,6 +30,11 @@ HOST_SYMBOLS = [
     "mdkh_autogrid", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_run_table", "mdkh_last_error",
 ]
 STAGES = ["voxelize", "fwd_z", "fwd_y", "x_corr", "inv_y", "inv_z_top", "reduce"]
+# FFT sizes with compile-time specialised kernels (megadock_amd/csrc/mdk_pipe.hip): the
+# tuned sizes, with every kernel variant, and the further sizes, default variants only;
+# other sizes run the generic mixed-radix kernels
+TUNED_NF = [80, 96, 144, 256]
+FURTHER_NF = [64, 66, 70, 72, 84, 88, 90, 98, 100, 108, 110, 112, 120, 126, 128, 132, 140, 160, 176, 192]
 
 _lib = None
 

@@ -1,804 +1,14 @@
-// FFT-side docking pipeline specialised at compile time for the common grid sizes.
-// Same data flow and buffers as the generic kernels in mdk_dock.hip (see there), but
-// every 1D transform is a four-step register FFT (mdk_regfft.h) with a single LDS
-// exchange, and the correlation pass reuses each receptor tile for RB rotations.
-#include <hip/hip_runtime.h>
+// Specialised pipeline: the tuned sizes (every kernel variant, for the autotuner) and
+// the dispatch to the further sizes built one per object file from mdk_pipe_size.hip.
+#include "mdk_pipe_kernels.h"
 
-#include <cmath>
-
-#include "mdk_internal.h"
-#include "mdk_regfft.h"
-
-namespace {
-
-using rf::Geo;
-using rf::sfor;
-
-// Global accesses as a block-uniform base plus a 32-bit byte offset per lane, so that
-// the address is formed by the load/store itself (scalar base + vector offset) instead
-// of 64-bit vector arithmetic per access.
-template <class T> __device__ __forceinline__ T ld_at(const T *base, unsigned byte_off) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-template <class T> __device__ __forceinline__ void st_at(T *base, unsigned byte_off, const T &v) {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
-}
-
-
-// ------------------------------------------------------------------ forward z
-// lines (b, x, y), x, y < ng: the ng ligand voxels of the row sit at z = nd .. nd+ng-1;
-// their values come straight from the voxel masks and the charge grid into the step-1
-// registers.  Thread (line, u) with u fastest, so a line's voxels go to neighbouring lanes.
-template <int N, int L>
-__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_z(VoxGrid vg, float2 *__restrict__ a1,
-                                                          const float2 *__restrict__ twg, int nlines) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    constexpr int NG = N / 2, ND = N / 4;  // ligand voxels per axis and their offset in the line
-    __shared__ float2 S[N * LS];
-    __shared__ float2 tw[N];
-    const int tid = threadIdx.x, line = tid / T, u = tid - line * T;
-    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int gl = blockIdx.x * L + line;
-    const bool act = gl < nlines;
-    const int b = gl / (NG * NG), row = gl - b * (NG * NG);
-    __syncthreads();
-    if (u < N2) {
-        float2 v[N1];
-        sfor<0, N1>([&](auto jc) {
-            constexpr int j1 = decltype(jc)::value;
-            const int e = u + N2 * j1;
-            v[j1] = (act && e >= ND && e < ND + NG) ? vox_take(vg, b, row, e - ND) : make_float2(0.f, 0.f);
-        });
-        rf::step1_store<N, false>(v, S, tw, LS, line, u);
-    }
-    __syncthreads();
-    if (u < N1) {
-        float2 y[N2];
-        rf::step2_load<N, false>(y, S, LS, line, u);
-        float2 *dst = a1 + (size_t)blockIdx.x * L * N;  // uniform base, 32-bit lane offsets
-        if (act)
-            sfor<0, N2>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                st_at(dst, 8u * (unsigned)(line * N + u + N1 * k2), y[k2]);
-            });
-    }
-}
-
-// ------------------------------------------------------------------ forward y
-// block (kz chunk, x, b); rows y < ng of a1 sit at y + nd; loads straight to registers
-template <int N, int L>
-__global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restrict__ a1, float2 *__restrict__ a2,
-                                                          const float2 *__restrict__ twg) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    constexpr int NG = N / 2, ND = N / 4;
-    __shared__ float2 S[N * LS];
-    __shared__ float2 tw[N];
-    const int tid = threadIdx.x, line = tid % L, u = tid / L;
-    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int kz = blockIdx.x * L + line, x = blockIdx.y, b = blockIdx.z;
-    const bool col = kz < N;
-    const float2 *src = a1 + ((size_t)b * NG + x) * NG * N;
-    __syncthreads();
-    if (u < N2) {
-        float2 v[N1];
-        sfor<0, N1>([&](auto jc) {
-            constexpr int j1 = decltype(jc)::value;
-            const int y = u + N2 * j1 - ND;
-            v[j1] = (col && y >= 0 && y < NG) ? ld_at(src, 8u * (unsigned)(y * N + kz)) : make_float2(0.f, 0.f);
-        });
-        rf::step1_store<N, false>(v, S, tw, LS, line, u);
-    }
-    __syncthreads();
-    if (u < N1) {
-        float2 y[N2];
-        rf::step2_load<N, false>(y, S, LS, line, u);
-        float2 *dst = a2 + ((size_t)b * NG + x) * N * N;
-        if (col)
-            sfor<0, N2>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                st_at(dst, 8u * (unsigned)((u + N1 * k2) * N + kz), y[k2]);
-            });
-    }
-}
-
-// ------------------------------------------------------------------ forward z + y, fused
-// Persistent workgroups walking the (b, x) planes, x < ng, of the ligand.  Per plane:
-//   z-FFT of the ng rows y (voxel values straight from the masks and the charge grid into
-//   the step-1 registers), kept in LDS as A[kz][y];
-//   y-FFT of the nf columns kz in chunks of C, the ng non-zero rows at y + nd read from A,
-//   the outputs written straight to a2[b][x][ky][kz].
-// It replaces kp_fwd_z + kp_fwd_y and never writes a1.
-template <int N, int C> constexpr bool fwd_zy_fits() {
-    constexpr int NG = N / 2, T = Geo<N>::T;
-    return NG * T <= 1024 && C * T <= 1024 && N % C == 0 &&
-           (size_t)N * (NG + 1) * 8 + (size_t)N * (C + 1) * 8 + (size_t)N * 8 <= 150 * 1024;
-}
-
-template <int N, int C>
-__global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict__ a2, const float2 *__restrict__ twg,
-                                                   int nplanes) {
-    constexpr int NG = N / 2, ND = N / 4;
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
-    constexpr int LA = NG + 1, LE = C + 1;
-    static_assert(fwd_zy_fits<N, C>(), "fused forward pass does not fit");
-    __shared__ float2 A[N * LA];  // z-transformed plane, [kz][y]
-    __shared__ float2 E[N * LE];  // y-exchange of one chunk of columns
-    __shared__ float2 tw[N];
-    for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
-    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
-        const int b = p / NG, x = p - b * NG;
-        // opaque thread index: keeps the per-thread addresses out of the plane loop
-        int tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));
-        __syncthreads();  // A, E (and tw) free
-        // ---- z-FFT of row y: thread (y, u), u fastest so that a row's voxels sit in neighbouring lanes
-        {
-            const int y = tid / T, u = tid - y * T;
-            const bool zrow = y < NG;
-            const int row = x * NG + y;
-            if (zrow && u < N2) {
-                // the row's mask words once, then one bit per voxel; the charges one by one
-                constexpr int RW = (NG + 31) / 32;
-                static_assert(RW <= 4, "row of at most 128 voxels");
-                const uint32_t *dr = vg.dmo + (size_t)b * 2 * vg.mw + (size_t)row * RW;
-                const int mw = vg.mw;
-                const uint32_t d0 = dr[0], o0 = dr[mw];
-                const uint32_t d1 = RW > 1 ? dr[1] : 0u, o1 = RW > 1 ? dr[mw + 1] : 0u;
-                const uint32_t d2 = RW > 2 ? dr[2] : 0u, o2 = RW > 2 ? dr[mw + 2] : 0u;
-                const uint32_t d3 = RW > 3 ? dr[3] : 0u, o3 = RW > 3 ? dr[mw + 3] : 0u;
-                float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)row * NG;
-                const float delta = vg.delta;
-                float2 v[N1] = {};
-#pragma unroll
-                for (int j1 = 0; j1 < N1; j1++) {
-                    const int e = u + N2 * j1;
-                    if (e >= ND && e < ND + NG) {
-                        const int k = e - ND, w = k >> 5;
-                        const uint32_t d = w == 0 ? d0 : w == 1 ? d1 : w == 2 ? d2 : d3;
-                        const uint32_t o = w == 0 ? o0 : w == 1 ? o1 : w == 2 ? o2 : o3;
-                        const uint32_t bit = 1u << (k & 31);
-                        const float q = cr[k];
-                        if (__float_as_uint(q) != 0u) cr[k] = 0.0f;  // keep the charge grid zero
-                        v[j1] = make_float2((d & bit) ? delta : ((o & bit) ? 1.0f : 0.0f), q);
-                    }
-                }
-                rf::step1_g<N, false>(v, A, tw, 1, LA, 1, y, u);
-            }
-            __syncthreads();
-            float2 w[N2] = {};
-            if (zrow && u < N1) rf::step2_g<N, false>(w, A, LA, 1, y, u);
-            __syncthreads();
-            if (zrow && u < N1)
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    A[(u + N1 * k2) * LA + y] = w[k2];
-                });
-        }
-        __syncthreads();
-        // ---- y-FFT of the columns, C at a time: thread (column, u), column fastest
-        float2 *dst = a2 + (size_t)p * N * N;  // a2[b][x]: [ky][kz]
-        const int cl = tid % C, u = tid / C;
-        for (int c0 = 0; c0 < N; c0 += C) {  // uniform
-            const int kz = c0 + cl;
-            if (u < N2) {
-                float2 v[N1] = {};
-                sfor<0, N1>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    const int yy = u + N2 * j1 - ND;
-                    if (yy >= 0 && yy < NG) v[j1] = A[kz * LA + yy];
-                });
-                rf::step1_g<N, false>(v, E, tw, 1, LE, 1, cl, u);
-            }
-            __syncthreads();
-            if (u < N1) {
-                float2 w[N2];
-                rf::step2_g<N, false>(w, E, LE, 1, cl, u);
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    st_at(dst, 8u * (unsigned)((u + N1 * k2) * N + kz), w[k2]);
-                });
-            }
-            __syncthreads();  // E is rewritten by the next chunk
-        }
-    }
-}
-
-// ------------------------------------------------------------------ x + correlation
-// block (kz chunk c, ky, rotation group).  Lines 0..L-1: (ky, kz = c L + l); lines
-// L..2L-1: partner columns (-ky, -kz).  Forward x-FFT of all 2L lines, the Hermitian
-// fold Q(k) = (conj(R(k)) L(k) + conj(conj(R(-k)) L(-k))) / 2, inverse x-FFT of Q.
-// The receptor values of the tile are loaded once and reused for RB rotations.
-// B_FAST: the rotation is the fastest grid index, so the workgroups that share a
-// receptor tile run back to back and find it in L2 / the last-level cache.
-// QA: the folded lines Q share the LDS of S (two more barriers, a third less LDS).
-// SPEC: store the folded spectrum Q itself, cc[b][kx][ky][kz], and skip the inverse x-FFT
-// (the spectral score evaluation reads it).
-template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false, bool SPEC = false>
-__global__ void __launch_bounds__(2 * L *Geo<N>::T)
-    kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
-              const float2 *__restrict__ twg, int H, int Hp, int B) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
-    constexpr int NG = N / 2, ND = N / 4;
-    constexpr int L2 = 2 * L, LS2 = L2 + 1, LS = L + 1, NT = L2 * T;
-    constexpr int PER = (L * N + NT - 1) / NT;  // fold elements per thread
-    __shared__ float2 S[N * LS2];
-    __shared__ float2 Qs[QA ? 1 : N * LS];
-    __shared__ float2 tw[N];
-    float2 *const Q = QA ? S : Qs;
-    const int tid = threadIdx.x;
-    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int c = B_FAST ? blockIdx.y : blockIdx.x, ky = B_FAST ? blockIdx.z : blockIdx.y;
-    const int b0 = (B_FAST ? blockIdx.x : blockIdx.z) * RB;
-    const int kyp = (N - ky) % N;
-    // fold elements of this thread: column l (fixed), rows kx = fx0 + FX p, p < PER
-    static_assert(NT % L == 0, "fold mapping");
-    constexpr int FX = NT / L;
-    const int fl = tid % L, fx0 = tid / L, fkz = c * L + fl;
-    const bool fcol = fkz < H;
-    const unsigned fa0 = 8u * (unsigned)((fx0 * N + ky) * N + fkz);                    // + 8 FX N^2 p
-    const unsigned fd0 = 8u * (unsigned)(((N - fx0) * N + kyp) * N + (N - fkz) % N);  // - 8 FX N^2 p
-    // receptor values of this thread's fold elements (kept in registers when CACHE_R)
-    auto rec_pair = [&](int p, float2 &a, float2 &d) {
-        a = d = make_float2(0.f, 0.f);
-        if (fx0 + FX * p < N && fcol) {
-            a = ld_at(rec, fa0 + 8u * (unsigned)(FX * p * N * N));
-            // partner row -kx: N - kx for kx > 0 and 0 for kx = 0
-            const unsigned od = (fx0 + FX * p == 0) ? fd0 - 8u * (unsigned)(N * N * N) : fd0 - 8u * (unsigned)(FX * p * N * N);
-            d = ld_at(rec, od);
-        }
-    };
-    float2 r1[CACHE_R ? PER : 1], r2[CACHE_R ? PER : 1];
-    if constexpr (CACHE_R) {
-#pragma unroll
-        for (int p = 0; p < PER; p++) rec_pair(p, r1[p], r2[p]);
-    }
-    const int line = tid % L2, u = tid / L2;
-    const int kzl = c * L + (line % L);
-    const bool valid_line = kzl < H;
-    const int kyl = line < L ? ky : kyp;
-    const int kzs = line < L ? kzl : (N - kzl) % N;
-#pragma unroll 1
-    for (int bb = 0; bb < RB; bb++) {
-        const int b = b0 + bb;
-        if (b >= B) break;  // uniform across the block
-        const float2 *src = a2 + (size_t)b * NG * N * N;  // uniform base, 32-bit lane offsets
-        __syncthreads();  // S, Q free (previous rotation done)
-        if (u < N2) {
-            float2 v[N1];
-            const unsigned o0 = 8u * (unsigned)((u - ND) * N * N + kyl * N + kzs);  // used only when x is in range
-            sfor<0, N1>([&](auto jc) {
-                constexpr int j1 = decltype(jc)::value;
-                const int x = u + N2 * j1 - ND;
-                v[j1] = (valid_line && x >= 0 && x < NG) ? ld_at(src, o0 + 8u * N2 * j1 * N * N) : make_float2(0.f, 0.f);
-            });
-            rf::step1_store<N, false>(v, S, tw, LS2, line, u);
-        }
-        __syncthreads();
-        float2 y[N2];
-        if (u < N1) rf::step2_load<N, false>(y, S, LS2, line, u);
-        __syncthreads();
-        if (u < N1)
-            sfor<0, N2>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                S[(u + N1 * k2) * LS2 + line] = y[k2];
-            });
-        __syncthreads();
-        // Hermitian fold into Q
-        float2 qv[PER];
-#pragma unroll
-        for (int p = 0; p < PER; p++) {
-            const int kx = fx0 + FX * p;
-            if (kx < N) {
-                const int kxp = kx == 0 ? 0 : N - kx;
-                const float2 l1 = S[kx * LS2 + fl], l2 = S[kxp * LS2 + L + fl];
-                float2 a, d;
-                if constexpr (CACHE_R) {
-                    a = r1[p];
-                    d = r2[p];
-                } else {
-                    rec_pair(p, a, d);
-                }
-                const float2 p1 = make_float2(a.x * l1.x + a.y * l1.y, a.x * l1.y - a.y * l1.x);
-                const float2 p2 = make_float2(d.x * l2.x + d.y * l2.y, d.x * l2.y - d.y * l2.x);
-                qv[p] = make_float2(0.5f * (p1.x + p2.x), 0.5f * (p1.y - p2.y));
-            }
-        }
-        if constexpr (SPEC) {
-            float2 *dst = cc + (size_t)b * N * N * Hp;
-            if (fkz < Hp)
-#pragma unroll
-                for (int p = 0; p < PER; p++)
-                    if (fx0 + FX * p < N) st_at(dst, 8u * (unsigned)(((fx0 + FX * p) * N + ky) * Hp + fkz), qv[p]);
-            continue;
-        }
-        if (QA) __syncthreads();  // every read of S above is done before Q overwrites it
-#pragma unroll
-        for (int p = 0; p < PER; p++)
-            if (fx0 + FX * p < N) Q[(fx0 + FX * p) * LS + fl] = qv[p];
-        __syncthreads();
-        // inverse x-FFT of the L lines of Q; thread (lq, uq) over the first L*T threads
-        const int lq = tid % L, uq = tid / L;
-        const bool act = tid < L * T;
-        {
-            float2 v[N1];
-            if (act && uq < N2)
-                sfor<0, N1>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    v[j1] = Q[(uq + N2 * j1) * LS + lq];
-                });
-            if (QA) __syncthreads();  // Q is read before step 1 overwrites it
-            if (act && uq < N2) rf::step1_store<N, true>(v, S, tw, LS, lq, uq);
-        }
-        __syncthreads();
-        if (act && uq < N1) {
-            float2 z[N2];
-            rf::step2_load<N, true>(z, S, LS, lq, uq);
-            const int kz = c * L + lq;
-            if (kz < Hp) {
-                float2 *dst = cc + (size_t)b * N * N * Hp;  // uniform base, 32-bit lane offsets
-                const unsigned o0 = 8u * (unsigned)((uq * N + ky) * Hp + kz);
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    st_at(dst, o0 + 8u * (unsigned)(N1 * k2 * N * Hp), z[k2]);
-                });
-            }
-        }
-    }
-}
-
-// ------------------------------------------------------------------ inverse y, in place
-template <int N, int L>
-__global__ void __launch_bounds__(L *Geo<N>::T) kp_inv_y(float2 *__restrict__ cc, const float2 *__restrict__ twg, int Hp) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    __shared__ float2 S[N * LS];
-    __shared__ float2 tw[N];
-    const int tid = threadIdx.x, line = tid % L, u = tid / L;
-    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int kz = blockIdx.x * L + line, x = blockIdx.y, b = blockIdx.z;
-    float2 *base = cc + ((size_t)b * N + x) * N * Hp;
-    __syncthreads();
-    if (u < N2) {
-        float2 v[N1];
-        sfor<0, N1>([&](auto jc) {
-            constexpr int j1 = decltype(jc)::value;
-            v[j1] = base[(size_t)(u + N2 * j1) * Hp + kz];
-        });
-        rf::step1_store<N, true>(v, S, tw, LS, line, u);
-    }
-    __syncthreads();
-    if (u < N1) {
-        float2 y[N2];
-        rf::step2_load<N, true>(y, S, LS, line, u);
-        sfor<0, N2>([&](auto kc) {
-            constexpr int k2 = decltype(kc)::value;
-            base[(size_t)(u + N1 * k2) * Hp + kz] = y[k2];
-        });
-    }
-}
-
-// Block-wide top-t: each wavefront extracts the top-t of its lanes' lists, lane 0 of each
-// wavefront leaves them in wl, and thread 0 merges the NW short lists and writes t
-// candidates to dst.
-template <int TT, int NW>
-__device__ __forceinline__ void block_top(TopR<TT> &tl, int t, Cand *wl, Cand *dst) {
-    float ov[TT];
-    int oi[TT];
-    wave_top<TT>(tl, t, ov, oi);
-    const int tid = threadIdx.x;
-    if ((tid & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < TT; q++)
-            if (q < t) wl[(tid >> 6) * TT + q] = Cand{ov[q], oi[q]};
-    __syncthreads();
-    if (tid == 0) {
-        TopR<TT> m;
-        m.init();
-        for (int w = 0; w < NW; w++)
-#pragma unroll
-            for (int q = 0; q < TT; q++)
-                if (q < t) m.push(wl[w * TT + q].key, wl[w * TT + q].idx);
-#pragma unroll
-        for (int q = 0; q < TT; q++)
-            if (q < t) dst[q] = Cand{m.v[q], m.i[q]};
-    }
-}
-
-// ------------------------------------------------------------------ inverse z + top-t
-// block (x, b): the N rows r = x*N + y of plane x of rotation b, L rows at a time; the
-// row's spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps a
-// running top-TT (TT >= t, compile time) of its outputs over the plane's row groups; the
-// lists are merged across each wavefront with lane shuffles and across wavefronts through
-// LDS once per plane, leaving the plane's top-t as t candidates.  The test hook writes the
-// real score rows to smap instead.
-template <int N, int L, int TT>
-__global__ void __launch_bounds__(L *Geo<N>::T)
-    kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
-                 const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
-    constexpr int NROW = N * N, NW = (NT + 63) / 64, RG = N / L;
-    static_assert(NT % 64 == 0, "the wavefront top-t needs whole wavefronts");
-    static_assert(N % L == 0, "whole row groups per plane");
-    __shared__ float2 S[N * LS];
-    __shared__ float2 tw[N];
-    __shared__ Cand wl[NW * TT];
-    const int tid = threadIdx.x, line = tid % L, u = tid / L;
-    for (int q = tid; q < N; q += NT) tw[q] = twg[q];
-    const int b = blockIdx.y;
-    const float2 *src = cc + ((size_t)b * NROW) * Hp;
-    // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
-    // >= tau (the t largest lane maxima of a row group, the largest such bound over the
-    // plane so far, taken from its first four row groups and every fourth after; the t
-    // scores behind it were pushed when it was raised), so anything whose key falls below
-    // tau's loses.  Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1
-    // exactly when t > 1).
-    constexpr bool QUOT = TT > 1;
-    TopR<TT> tl;
-    tl.init();
-    float run_tau = -INFINITY, cut = -INFINITY;
-    for (int g = 0; g < RG; g++) {
-        const int row0 = (blockIdx.x * RG + g) * L;
-        __syncthreads();  // S is free: the previous row group has read it
-        for (int q = tid; q < L * H; q += NT) {
-            const int l = q / H, kz = q - l * H;
-            const float2 v = src[(size_t)(row0 + l) * Hp + kz];
-            S[kz * LS + l] = v;
-            if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
-        }
-        __syncthreads();
-        {
-            float2 v[N1];
-            if (u < N2)
-                sfor<0, N1>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    v[j1] = S[(u + N2 * j1) * LS + line];
-                });
-            __syncthreads();  // step 1 writes back into S
-            if (u < N2) rf::step1_store<N, true>(v, S, tw, LS, line, u);
-        }
-        __syncthreads();
-        const int r = row0 + line;
-        const bool act = u < N1;
-        float2 y[N2];
-        if (u < N1) rf::step2_load<N, true>(y, S, LS, line, u);
-        if (smap) {  // uniform
-            if (act)
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
-                });
-            continue;
-        }
-        if (g < 4 || (g & 3) == 0) {  // uniform
-            float mx = -INFINITY;
-            if (act)
-                sfor<0, N2>([&](auto kc) { mx = fmaxf(mx, y[decltype(kc)::value].x); });
-            const float tau = wave_kth_max(mx, t);
-            if (tau > run_tau) {  // uniform
-                run_tau = tau;
-                cut = QUOT ? quotient_cut(tau, nf3f) : tau;
-            }
-        }
-        if (act)
-            sfor<0, N2>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                const float v = y[k2].x;
-                if (v >= cut) tl.push(QUOT ? v / nf3f : v, r * N + u + N1 * k2);
-            });
-    }
-    if (smap) return;  // uniform
-    block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
-}
-
-template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
-    constexpr int T = Geo<N>::T;
-    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), L * T, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, a.t, a.nf3f);
-}
-
-// the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS
-constexpr int plane_stride(int n) { return ((n / 2 + 1) % 2) ? n / 2 + 1 : n / 2 + 2; }
-constexpr bool fused_inverse(int n) { return (size_t)n * plane_stride(n) * 8 <= 150 * 1024; }
-
-// ------------------------------------------------------------------ fused inverse y + z + top-t
-// Persistent workgroups, each walking the (b, x) planes p = b*nf + x of cc (nf rows ky,
-// kz <= nf/2 columns) with stride gridDim.x.  Per plane, held in LDS:
-//   y-IFFT of every kz column in place (step-1 inputs come straight from HBM into
-//   registers; the next plane's are loaded while the current one is computed);
-//   per row, the half-length complex-to-real step: with M = nf/2 and w = exp(+2 pi i / nf),
-//     Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])),  k < M,
-//   whose M-point inverse gives z[m] = x[2m] + i x[2m+1] (x the real score row);
-//   the M-point inverse of each row, whose outputs go straight into the top-t lists.
-// The top-t is written as one candidate list per plane.  PF: prefetch the next plane's
-// step-1 inputs (in flight during the current plane) or load them at the start.
-// SEG: each workgroup walks a contiguous range of planes instead; the planes of one
-// rotation inside the range form a segment whose lane lists, and whose running threshold,
-// carry over from plane to plane.  The segment's top-t goes to the list of its first plane
-// and the lists of its other planes are left empty, so the wavefront merge and the
-// block merge run once per segment rather than once per plane.
-template <int N, int TT, bool PF, bool SEG = false>
-__global__ void __launch_bounds__(1024)
-    kp_inv_yz_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
-                  const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
-    constexpr int NTH = 1024, NW = NTH / 64;
-    constexpr int M = N / 2, H = M + 1;
-    constexpr int LSP = plane_stride(N);  // row stride of the plane (odd)
-    constexpr int N1y = Geo<N>::N1, N2y = Geo<N>::N2;
-    constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2;
-    constexpr int KZ = (N * N1z + NTH - 1) / NTH;  // z tasks per thread
-    static_assert(H * N2y <= NTH && H * N1y <= NTH, "one y task per thread");
-    __shared__ float2 P[N * LSP];
-    __shared__ float2 tw[N];
-    __shared__ Cand wl[NW * TT];
-    const int tid = threadIdx.x;
-    for (int q = tid; q < N; q += NTH) tw[q] = twg[q];
-    // y task of a thread: column kz = yl, step-1 index yu (< N2y), step-2 index yu (< N1y)
-    // planes of this workgroup: p_first, p_first + p_step, ... below p_end
-    int p_first = blockIdx.x, p_end = nplanes, p_step = gridDim.x;
-    if constexpr (SEG) {
-        const int chunk = (nplanes + gridDim.x - 1) / gridDim.x;
-        p_first = blockIdx.x * chunk;
-        p_end = min(nplanes, p_first + chunk);
-        p_step = 1;
-    }
-    float2 vn[N1y];  // step-1 inputs of the plane to come
-    auto load_plane = [&](int p, int yl, int yu) {
-        if (yu < N2y && p < p_end) {
-            const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
-            const unsigned off = (unsigned)(yu * Hp + yl);
-            sfor<0, N1y>([&](auto jc) {
-                constexpr int j1 = decltype(jc)::value;
-                vn[j1] = src[off + (unsigned)(N2y * j1 * Hp)];
-            });
-        }
-    };
-    constexpr bool QUOT = TT > 1;
-    // SEG: the running lane lists, the running threshold and the segment's first plane
-    TopR<TT> tl;
-    tl.init();
-    float run_tau = -INFINITY, cut = -INFINITY;
-    int seg0 = p_first;
-    if (PF) load_plane(p_first, tid % H, tid / H);
-    for (int p = p_first; p < p_end; p += p_step) {  // uniform across the block
-        const int b = p / N, x = p - b * N;
-        // The thread's LDS addresses are recomputed in every iteration: an opaque copy of
-        // the thread index keeps them from being hoisted out of the plane loop, where they
-        // would all stay live and spill.
-        int tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));
-        const int yl = tid % H, yu = tid / H;
-        if (!PF) load_plane(p, yl, yu);
-        __syncthreads();  // P, tw and wl are free: the previous plane is done with them
-        // ---- y-IFFT of the H columns (line = kz, element = ky): element stride LSP
-        if (yu < N2y) {
-            float2 v[N1y];
-            sfor<0, N1y>([&](auto jc) { v[decltype(jc)::value] = vn[decltype(jc)::value]; });
-            rf::step1_g<N, true>(v, P, tw, 1, LSP, 1, yl, yu);
-        }
-        if (PF) load_plane(p + p_step, yl, yu);
-        __syncthreads();
-        {
-            float2 y[N2y];
-            if (yu < N1y) rf::step2_g<N, true>(y, P, LSP, 1, yl, yu);
-            __syncthreads();
-            if (yu < N1y)
-                sfor<0, N2y>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    P[(yu + N1y * k2) * LSP + yl] = y[k2];
-                });
-        }
-        __syncthreads();
-        // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP),
-        // its inputs formed by the complex-to-real pre-step as they are loaded:
-        // Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])), w^k = conj(tw[k])
-        // zero-initialised so that no value of the arrays is carried around the plane loop
-        float2 y[KZ][N2z] = {};
-        {
-            float2 v[KZ][N1z] = {};
-#pragma unroll
-            for (int j = 0; j < KZ; j++) {
-                const int task = tid + j * NTH, line = task % N, u = task / N;
-                if (u < N2z)
-                    sfor<0, N1z>([&](auto jc) {
-                        constexpr int j1 = decltype(jc)::value;
-                        const int k = u + N2z * j1;
-                        const float2 a = P[line * LSP + k], c = P[line * LSP + M - k], w = tw[k];
-                        const float2 s1 = make_float2(a.x + c.x, a.y - c.y);  // X[k] + conj(X[M-k])
-                        const float2 d1 = make_float2(a.x - c.x, a.y + c.y);  // X[k] - conj(X[M-k])
-                        // e = conj(w) d1; Z = s1 + i e
-                        const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
-                        v[j][j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
-                    });
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < KZ; j++) {
-                const int task = tid + j * NTH, line = task % N, u = task / N;
-                if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < KZ; j++) {
-                const int task = tid + j * NTH, line = task % N, u = task / N;
-                if (u < N1z) rf::step2_g<M, true>(y[j], P, 1, LSP, line, u);
-            }
-        }
-        // z = 2m and 2m+1 of row `line` are y[j][k2].x and .y with m = u + N1z k2
-        if (smap) {  // uniform
-#pragma unroll
-            for (int j = 0; j < KZ; j++) {
-                const int task = tid + j * NTH, line = task % N, u = task / N;
-                if (u < N1z)
-                    sfor<0, N2z>([&](auto kc) {
-                        constexpr int k2 = decltype(kc)::value;
-                        const size_t o = (size_t)b * N * N * N + (size_t)(x * N + line) * N + 2 * (u + N1z * k2);
-                        smap[o] = y[j][k2].x;
-                        smap[o + 1] = y[j][k2].y;
-                    });
-            }
-            continue;
-        }
-        // Only scores that can reach the wavefront's top-t are pushed: at least t scores are
-        // >= tau (the t largest lane maxima), so anything whose key falls below tau's loses.
-        // Keys: raw values for t == 1, fl(value / nf^3) for t > 1 (TT > 1 exactly when t > 1).
-        // SEG: tau is the largest such bound over the segment's planes so far, taken from
-        // its first four planes and every eighth after; the t scores behind it were pushed
-        // when it was raised and can only be displaced by better ones.
-        const int k = p - seg0;
-        if (!SEG || k < 4 || (k & 7) == 0) {  // uniform
-            float mx = -INFINITY;
-#pragma unroll
-            for (int j = 0; j < KZ; j++) {
-                if ((tid + j * NTH) / N < N1z)
-                    sfor<0, N2z>([&](auto kc) {
-                        constexpr int k2 = decltype(kc)::value;
-                        mx = fmaxf(mx, fmaxf(y[j][k2].x, y[j][k2].y));
-                    });
-            }
-            const float tau = wave_kth_max(mx, t);
-            if (!SEG || tau > run_tau) {  // uniform
-                run_tau = tau;
-                cut = QUOT ? quotient_cut(tau, nf3f) : tau;
-            }
-        }
-        if (!SEG) tl.init();
-#pragma unroll
-        for (int j = 0; j < KZ; j++) {
-            const int task = tid + j * NTH, line = task % N, u = task / N;
-            if (u < N1z) {
-                const int base = (x * N + line) * N + 2 * u;
-                sfor<0, N2z>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    const float v0 = y[j][k2].x, v1 = y[j][k2].y;
-                    if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, base + 2 * N1z * k2);
-                    if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, base + 2 * N1z * k2 + 1);
-                });
-            }
-        }
-        if constexpr (SEG) {
-            if (p != seg0 && tid < t) cand[(size_t)p * t + tid] = Cand{-INFINITY, kNoIndex};  // empty list
-            if (p + 1 == p_end || (p + 1) / N != b) {  // uniform: the segment ends with this plane
-                block_top<TT, NW>(tl, t, wl, cand + (size_t)seg0 * t);
-                tl.init();
-                run_tau = cut = -INFINITY;
-                seg0 = p + 1;
-            }
-        } else {
-            block_top<TT, NW>(tl, t, wl, cand + (size_t)p * t);
-        }
-    }
-}
-
-template <int N, int TT> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
-    const int nplanes = a.B * N;
-    const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
-    switch (a.inv_variant) {
-        case 1:
-            kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
-            break;
-        case 2:
-            kp_inv_yz_top<N, TT, false, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f,
-                                                                  nplanes);
-            break;
-        case 3:
-            kp_inv_yz_top<N, TT, true, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f,
-                                                                 nplanes);
-            break;
-        default:
-            kp_inv_yz_top<N, TT, false><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
-    }
-}
-
-// columns per chunk of the fused forward pass, or 0 where it does not fit (then z and y
-// run as two passes)
-constexpr int fwd_zy_chunk(int n) {
-    return n == 80 ? 80 : n == 96 ? 48 : n == 144 ? 48 : 0;
-}
-
-template <int N> hipError_t run_pipe(const PipeArgs &a) {
-    constexpr int T = Geo<N>::T;
-    constexpr int L = 16, LX = 8;
-    const int B = a.B, ng = a.ng;
-    if (ng != N / 2 || a.nd != N / 4) return hipErrorInvalidValue;  // the kernels assume nf = 2 ng, nd = ng / 2
-    hipStream_t s = a.stream;
-    constexpr int CZY = fwd_zy_chunk(N);
-    bool fused_fwd = false;
-    if constexpr (CZY > 0) {
-        if (a.fwd_variant == 0) {  // fused z + y
-            const int nplanes = B * ng;
-            const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;
-            kp_fwd_zy<N, CZY><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
-            if (a.ev) (void)hipEventRecord(a.ev[0], s);  // the z stage is fused into the y stage
-            fused_fwd = true;
-        }
-    }
-    if (!fused_fwd) {
-        const int lines_z = B * ng * ng;
-        kp_fwd_z<N, L><<<(lines_z + L - 1) / L, L * T, 0, s>>>(a.vox, a.a1, a.tw4, lines_z);
-        if (a.ev) (void)hipEventRecord(a.ev[0], s);
-        kp_fwd_y<N, L><<<dim3((N + L - 1) / L, ng, B), L * T, 0, s>>>(a.a1, a.a2, a.tw4);
-    }
-    if (a.ev) (void)hipEventRecord(a.ev[1], s);
-    if (a.spectrum_only) {
-        kp_x_corr<N, LX, 1, false, true, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
-                                                                                                  a.H, a.Hp, B);
-        return hipGetLastError();
-    }
-    // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
-    // the grid, so consecutive workgroups share the receptor tile in L2, folded lines kept
-    // in the exchange buffer's LDS, 16 columns per workgroup with the receptor values
-    // loaded first (6, the default) or where the fold needs them (5), 8 columns (4); 8
-    // columns with a separate fold buffer
-    // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
-    // of 4 rotations (1); one rotation per workgroup, tile index fastest (2)
-    switch (a.variant) {
-        case 0:
-            kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-            break;
-        case 1:
-            kp_x_corr<N, LX, 4, false><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-            break;
-        case 2:
-            kp_x_corr<N, LX, 1, false><<<dim3(a.Hp / LX, N, B), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-            break;
-        case 3:
-            kp_x_corr<N, LX, 1, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-            break;
-        case 4:
-            kp_x_corr<N, LX, 1, false, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
-                                                                                              a.H, a.Hp, B);
-            break;
-        case 5:  // as 4 with 16 columns per workgroup
-            kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-            break;
-        default:  // 6: as 5 with the receptor tile loaded into registers before the forward x-FFT,
-                  // so that its latency hides behind the transform
-            kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
-    }
-    if (a.ev) (void)hipEventRecord(a.ev[2], s);
-    if constexpr (fused_inverse(N)) {
-        if (a.ev) (void)hipEventRecord(a.ev[3], s);  // inverse y is fused into the next pass
-        if (a.t <= 1) launch_inv_yz<N, 1>(a, s);
-        else if (a.t <= 2) launch_inv_yz<N, 2>(a, s);
-        else if (a.t <= 4) launch_inv_yz<N, 4>(a, s);
-        else if (a.t <= 8) launch_inv_yz<N, 8>(a, s);
-        else launch_inv_yz<N, 16>(a, s);
-    } else {
-        kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
-        if (a.ev) (void)hipEventRecord(a.ev[3], s);
-        if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
-        else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
-        else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
-        else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
-        else launch_inv_z<N, L, 16>(a, s);
-    }
-    if (a.ev) (void)hipEventRecord(a.ev[4], s);
-    return hipGetLastError();
-}
-
-}  // namespace
+// further sizes (mdk_pipe_size.hip, the Makefile's PIPE_SIZES)
+#define MDK_PIPE_SIZES(X) \
+    X(64) X(66) X(70) X(72) X(84) X(88) X(90) X(98) X(100) X(108) X(110) X(112) X(120) X(126) X(128) X(132) \
+    X(140) X(160) X(176) X(192)
+#define MDK_DECLARE(n) hipError_t pipe_run_n##n(const PipeArgs &a);
+MDK_PIPE_SIZES(MDK_DECLARE)
+#undef MDK_DECLARE
 
 int pipe_cands_per_rotation(int nf) {
     if (!pipe_available(nf)) return 0;
@@ -808,16 +18,23 @@ int pipe_cands_per_rotation(int nf) {
 bool pipe_available(int nf) {
     switch (nf) {
         case 80: case 96: case 144: case 256: return true;
+#define MDK_CASE(n) case n:
+        MDK_PIPE_SIZES(MDK_CASE)
+#undef MDK_CASE
+        return true;
         default: return false;
     }
 }
 
 hipError_t pipe_run(int nf, const PipeArgs &a) {
     switch (nf) {
-        case 80: return run_pipe<80>(a);
-        case 96: return run_pipe<96>(a);
-        case 144: return run_pipe<144>(a);
-        case 256: return run_pipe<256>(a);
+        case 80: return run_pipe<80, true>(a);
+        case 96: return run_pipe<96, true>(a);
+        case 144: return run_pipe<144, true>(a);
+        case 256: return run_pipe<256, true>(a);
+#define MDK_CASE(n) case n: return pipe_run_n##n(a);
+        MDK_PIPE_SIZES(MDK_CASE)
+#undef MDK_CASE
         default: return hipErrorInvalidValue;
     }
 }

@@ -136,7 +136,8 @@ def test_receptor_spectrum_vs_fp64(gpu, gcq):
 
 @pytest.mark.parametrize("which,ng", [("gcq", None), ("synth", 48), ("synth", 72), ("synth", 128), ("synth", 50)])
 def test_score_map_vs_fp64(gpu, gcq, synth, kernels, which, ng):
-    """nf = 80, 96, 144, 256 (specialised kernels exist) and 100 (generic only)."""
+    """nf = 80, 96, 144, 256 (tuned specialised kernels) and 100 (specialised with the
+    default variants; the generic kernels through the `kernels` fixture)."""
     if which == "gcq":
         rec, lig, ng = gcq
     else:
@@ -194,6 +195,46 @@ def test_dock_vs_oracle(gpu, gcq, synth, kernels, t, which, ng):
             ig = (g["index"][1] * nf + g["index"][2]) * nf + g["index"][3]
             ir = (r["index"][1] * nf + r["index"][2]) * nf + r["index"][3]
             assert _near_tie_ok(s64, ig, ir, 2e-6), (q, g, r, s64[ig] / nf3, s64[ir] / nf3)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nf", M.FURTHER_NF)
+def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf):
+    """The sizes specialised with the default kernel variants only (a table farm meets
+    them: nf = 2 ng from the reference's FFT size table, including ng odd, where the
+    ligand block sits at floor(ng/2)): score map vs FP64 and per-rotation top-2 vs the
+    oracle, as for the tuned sizes."""
+    rec, lig = synth
+    ng = nf // 2
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rr, ri = O.receptor_fft(ng, nf, gr, gi)
+    ctx = _lig_ctx(lig, ng, t=2, max_batch=8)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(3600)
+    remb = O.embed(ng, nf, gr, gi)
+    _say("further size nf=%d: score map" % nf)
+    a = 1234
+    smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
+    xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+    lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+    ref = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+    assert np.abs(smap - ref).max() <= 2e-6 * np.abs(ref).max()
+    ids = np.arange(7, 3600, 900 if nf >= 160 else 450, dtype=np.int32)
+    _say("further size nf=%d: dock %d rotations" % (nf, len(ids)))
+    got = ctx.dock(M.rotation_coefs(ang[ids]), ids)
+    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, 2, threads=16)
+    for q in range(len(ids) * 2):
+        g, r = got[q], exp[q]
+        assert g["index"][0] == r["index"][0]
+        assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
+        if not np.array_equal(g["index"], r["index"]):
+            b = int(ids[q // 2])
+            xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[b]))
+            lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+            s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+            ig = (g["index"][1] * nf + g["index"][2]) * nf + g["index"][3]
+            ir = (r["index"][1] * nf + r["index"][2]) * nf + r["index"][3]
+            assert _near_tie_ok(s64, ig, ir, 2e-6), (q, g, r)
     ctx.close()
 
 

@@ -56,3 +56,14 @@ def test_missing_table_is_an_error(tmp_path):
         M.run_table(["-tb", str(tmp_path / "none.table"), "-lg", str(tmp_path / "l.log"), "-dry"])
     with pytest.raises(M.MdkError, match="no job table"):
         M.run_table(["-lg", str(tmp_path / "l.log")])
+
+
+def test_specialised_size_lists_match_the_build():
+    """The Python lists of specialised sizes are the ones the library is built with."""
+    mk = open(os.path.join(ROOT, "megadock_amd", "csrc", "Makefile")).read()
+    sizes = [int(x) for x in re.search(r"^PIPE_SIZES := (.*)$", mk, re.M).group(1).split()]
+    assert sizes == M.FURTHER_NF
+    src = open(os.path.join(ROOT, "megadock_amd", "csrc", "mdk_pipe.hip")).read()
+    listed = [int(x) for x in re.findall(r"X\((\d+)\)", src)]
+    assert listed == M.FURTHER_NF
+    assert all("case %d: return run_pipe<%d, true>" % (n, n) in src for n in M.TUNED_NF)
