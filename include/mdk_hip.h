@@ -143,6 +143,17 @@ int mdk_score_spectral(mdk_ctx *ctx, const float *rotcoef, const float *t, int B
  * R)).  No reference counterpart: the differentiable companion used to refine poses. */
 int mdk_score_companion(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
 
+/* Real-space sums of the reference's detailed output (Docking::output_detail,
+ * src/docking.cpp:556-868) for B poses (any B; run in batches of the context's
+ * max_batch): rotation coefficients rotcoef (B*9, as mdk_dock)
+ * and grid translations trans (B*3, a decoy's index[1..3], within -ng .. 3ng-1).  The
+ * ligand is voxelized on the device; rec_rpsc (ng^3) is the receptor shape grid built
+ * without the ACE term (aceratio 0), the other receptor grids are mdk_set_receptor's.
+ * out (B*6, double): rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-), the float
+ * products of the reference summed in double.  Replaces the voxel loops of
+ * src/docking.cpp:596-657 and :710-801. */
+int mdk_score_detail(mdk_ctx *ctx, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, double *out);
+
 /* Measurement hook: device-to-device copy of `bytes` (read + write), `reps` times;
  * *ms = mean milliseconds per copy.  Calibrates HBM counters and the achievable copy
  * bandwidth.  No reference counterpart. */

@@ -22,7 +22,7 @@ ABI_SYMBOLS = [
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
-    "mdk_autotune", "mdk_set_ligand_atoms",
+    "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -205,6 +205,20 @@ class Context:
         assert len(rc) == len(tt)
         out = np.zeros((len(rc), 4), np.float32)
         _check(lib().mdk_score_spectral(self.h, _fp(rc), _fp(tt), C.c_int(len(rc)), _fp(out)), "mdk_score_spectral")
+        return out
+
+    def score_detail(self, rec_rpsc, rotcoef, trans):
+        """(B, 6) float64: real-space rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-) of
+        poses at grid translations trans (B, 3) int, as the reference's detailed output
+        sums them; rec_rpsc is the receptor shape grid built without the ACE term."""
+        rp = np.ascontiguousarray(rec_rpsc, np.float32).ravel()
+        assert rp.size == self.ng ** 3
+        rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
+        tr = np.ascontiguousarray(trans, np.int32).reshape(-1, 3)
+        out = np.zeros((len(rc), 6), np.float64)
+        _check(lib().mdk_score_detail(self.h, _fp(rp), _fp(rc), tr.ctypes.data_as(C.POINTER(C.c_int)),
+                                      C.c_int(len(rc)), out.ctypes.data_as(C.POINTER(C.c_double))),
+               "mdk_score_detail")
         return out
 
     def score_companion(self, rotcoef, t):

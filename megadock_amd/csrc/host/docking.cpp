@@ -178,6 +178,157 @@ int Docking::output(std::string &err) {
     return write_decoys(*prm_, *rec_, *lig_, num_grid_, fft_->top(), err);
 }
 
+namespace {
+
+// CA / backbone classes of src/protein.cpp:50-60: 1 for "CA ", 2 for "C " and "N ", else 0
+int atom_class(const std::string &key) {
+    if (key.compare(0, 3, "CA ") == 0) return 1;
+    if (key.compare(0, 2, "C ") == 0 || key.compare(0, 2, "N ") == 0) return 2;
+    return 0;
+}
+
+struct DetailRow {
+    float total, rpsc, rpsc_plus, rpsc_minus, elec, ace, rmsd_lca;
+    int near_native;
+};
+
+}  // namespace
+
+int write_detail(const Parameter &prm, const Receptor &rec, const Ligand &lig, int num_grid, const SortScore *top,
+                 mdk_ctx *ctx, std::string &err) {
+    const int ng1 = num_grid, nf1 = 2 * num_grid;
+    const size_t ng3 = (size_t)ng1 * ng1 * ng1;
+    const float gw = prm.grid_width;
+    const int natom = lig.num_atoms(), nout = prm.num_output;
+    // receptor shape grid without the ACE term (src/docking.cpp:568-571)
+    std::vector<float> rpsc(ng3), scratch(ng3);
+    {
+        std::vector<float> xyz(3 * (size_t)rec.num_atoms());
+        rec.shift_center(xyz.data());
+        rec.voxelize(ng1, gw, grid_coord(ng1, gw), xyz.data(), 0.0f, prm.rec_core, prm.elec_ratio, rpsc.data(),
+                     scratch.data());
+    }
+    float coordr[3], coordl[3], diff_dis[3];
+    int cor[3];
+    for (int i = 0; i < 3; i++) {  // src/docking.cpp:541-546
+        coordr[i] = rec.center(i);
+        coordl[i] = lig.center(i);
+        diff_dis[i] = coordr[i] - coordl[i];
+        cor[i] = (int)std::round(diff_dis[i] / gw);
+        if (cor[i] < 0) cor[i] += nf1;
+    }
+    // poses: the 7 around the input position (ligand unrotated, src/docking.cpp:596-615),
+    // then the decoys
+    const int np = 7 + nout;
+    std::vector<int> trans(3 * (size_t)np);
+    std::vector<float> theta(3 * (size_t)np, 0.0f), rot(9 * (size_t)np);
+    int c[3] = {cor[0], cor[1], cor[2]};
+    for (int rank = 0; rank < 7; rank++) {
+        if (rank == 1) c[0]--;
+        if (rank == 2) c[0] += 2;
+        if (rank == 3) { c[0]--; c[1]--; }
+        if (rank == 4) c[1] += 2;
+        if (rank == 5) { c[1]--; c[2]--; }
+        if (rank == 6) c[2] += 2;
+        for (int k = 0; k < 3; k++) trans[3 * rank + k] = c[k];
+    }
+    for (int rank = 0; rank < nout; rank++) {
+        const int a = top[rank].index[0];
+        for (int k = 0; k < 3; k++) {
+            theta[3 * (7 + rank) + k] = prm.zangle[3 * a + k];
+            trans[3 * (7 + rank) + k] = top[rank].index[k + 1];
+        }
+    }
+    if (mdk_rotation_coef(theta.data(), np, rot.data())) {
+        err = std::string("mdk_rotation_coef: ") + mdk_last_error();
+        return 1;
+    }
+    std::vector<double> sums(6 * (size_t)np);
+    if (mdk_score_detail(ctx, rpsc.data(), rot.data(), trans.data(), np, sums.data())) {
+        err = std::string("mdk_score_detail: ") + mdk_last_error();
+        return 1;
+    }
+    // RMSD of the translated (not rotated) ligand, as src/docking.cpp:663-693 and :829-851
+    std::vector<int> cls(natom);
+    int natom_lca = 0, natom_lbb = 0;
+    for (int i = 0; i < natom; i++) {
+        cls[i] = atom_class(lig.atom_type(i));
+        if (cls[i] == 1) natom_lca++;
+        if (cls[i] > 0) natom_lbb++;
+    }
+    std::vector<DetailRow> row(np);
+    for (int p = 0; p < np; p++) {
+        const double *d = &sums[6 * (size_t)p];
+        DetailRow &r = row[p];
+        r.rpsc_plus = (float)d[0];
+        r.rpsc_minus = (float)d[1];
+        r.rpsc = r.rpsc_plus + r.rpsc_minus;
+        const float elec = (float)d[2] + (float)d[3], ace = (float)d[4] + (float)d[5];
+        r.elec = elec;
+        r.ace = ace;
+        r.total = r.rpsc + r.elec + r.ace;
+        int tem[3];
+        for (int k = 0; k < 3; k++) tem[k] = (3 * ng1 - trans[3 * p + k]) % (2 * ng1);
+        float lca = 0.0f, lbb = 0.0f;
+        for (int i = 0; i < natom; i++)
+            for (int k = 0; k < 3; k++)
+                if (cls[i] > 0) {
+                    const float orig = lig.coordinate(i, k);
+                    const float moved = orig + gw * (tem[k] - ng1) + diff_dis[k];
+                    if (cls[i] == 1) lca += std::pow(moved - orig, 2.0);
+                    lbb += std::pow(moved - orig, 2.0);
+                }
+        r.rmsd_lca = std::sqrt(lca / natom_lca);
+        r.near_native = r.rmsd_lca < 5.0f ? 1 : 0;
+    }
+    FILE *fd = std::fopen(prm.detail_file.c_str(), "w");
+    if (!fd) {
+        err = "[ERROR] Output_detail file [" + prm.detail_file + "] not open!!";
+        return 1;
+    }
+    FILE *fc = std::fopen(prm.csv_file.c_str(), "w");
+    if (!fc) {
+        std::fclose(fd);
+        err = "[ERROR] Output_csv file [" + prm.csv_file + "] not open!!";
+        return 1;
+    }
+    const char *columns = "Rank   Angle               Trans         total       rPSC     rPSC(+)  rPSC(-)  ELEC       RecACE     LCARMSD NearNat";
+    std::fprintf(fd, "%d\t%.2f\n", nf1, gw);                 // src/docking.cpp:513-535
+    std::fprintf(fd, "\t%f\t%f\t%f\n", 0.0, 0.0, 0.0);       // receptor angle
+    std::fprintf(fd, "%s\t%f\t%f\t%f\n", prm.rec_file.c_str(), coordr[0], coordr[1], coordr[2]);
+    std::fprintf(fd, "%s\t%f\t%f\t%f\n", prm.lig_file.c_str(), coordl[0], coordl[1], coordl[2]);
+    std::fprintf(fd, "\nElec score ratio: %6.2f\nACE  score ratio: %6.2f\n", prm.elec_ratio, prm.ace_ratio);
+    std::fprintf(fd, "\ncorrect position: angle = (%5.2f,%5.2f,%5.2f), trans = (%3d,%3d,%3d) \n\n", 0.0, 0.0, 0.0,
+                 cor[0], cor[1], cor[2]);
+    std::fprintf(fd, " *** score around correct position *** \n\n%s\n", columns);
+    auto line = [&](int p, int rank, const float *th) {
+        const DetailRow &r = row[p];
+        std::fprintf(fd, "%5d: (%5.2f,%5.2f,%5.2f) (%3d,%3d,%3d) %11.2f %8.0f %8.0f %8.0f %10.2f %10.2f %7.3f %7d\n",
+                     rank, th[0], th[1], th[2], trans[3 * p], trans[3 * p + 1], trans[3 * p + 2], r.total, r.rpsc,
+                     r.rpsc_plus, r.rpsc_minus, r.elec, r.ace, r.rmsd_lca, r.near_native);
+    };
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    for (int rank = 0; rank < 7; rank++) line(rank, rank + 1, zero);
+    std::fprintf(fd, "\n *** docking results *** \n\n%s\n", columns);
+    std::fprintf(fc, "Rank,Anglex,Angley,Anglez,Transi,Transj,Transk,total,rPSC,rPSC_gain,rPSC_penalty,ELEC ,RecACE,LCARMSD,NearNat\n");
+    for (int rank = 0; rank < nout; rank++) {
+        const int p = 7 + rank;
+        float th[3];
+        for (int k = 0; k < 3; k++) {  // src/docking.cpp:819-823
+            th[k] = theta[3 * p + k];
+            if (th[k] >= PI_F) th[k] -= PI_F * 2.0;
+        }
+        line(p, rank + 1, th);
+        const DetailRow &r = row[p];
+        std::fprintf(fc, "%d, %f,%f,%f,%d,%d,%d,%f,%f,%f,%f,%f,%f,%f,%d\n", rank + 1, th[0], th[1], th[2], trans[3 * p],
+                     trans[3 * p + 1], trans[3 * p + 2], r.total, r.rpsc, r.rpsc_plus, r.rpsc_minus, r.elec, r.ace,
+                     r.rmsd_lca, r.near_native);
+    }
+    std::fclose(fd);
+    std::fclose(fc);
+    return 0;
+}
+
 // ------------------------------------------------------------------ run
 int run(int argc, char **argv, std::string &err) {
     const double t0 = now();
@@ -213,6 +364,9 @@ int run(int argc, char **argv, std::string &err) {
     const double t_init = now() - t0;
     std::printf("\n---------- Start docking calculations\n");
     if (dock.rec_init(err) || dock.dockz(err) || dock.output(err)) return 1;
+    if (prm.detail_output_flag &&  // src/control.cpp:486-490
+        write_detail(prm, rec, lig, dock.num_grid(), dock.fft_process()->top(), dock.fft_process()->ctx(), err))
+        return 1;
     std::printf("\nElapsed time  = %8.2f sec\n", now() - t0);
     std::printf(" Initialization = %8.2f sec\n Receptor process = %8.2f sec\n Docking = %8.2f sec (%d rotations, %.1f rotations/s)\n",
                 t_init, dock.t_receptor, dock.t_docking, prm.num_rot_angles, prm.num_rot_angles / dock.t_docking);

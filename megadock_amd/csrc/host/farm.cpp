@@ -346,6 +346,7 @@ int dock(Job &j, Contexts &cs) {
     std::vector<SortScore> top(best.size());
     std::memcpy(top.data(), best.data(), sizeof(SortScore) * best.size());
     if (write_decoys(p, *j.rec, L, j.ng, top.data(), j.err)) return 1;
+    if (p.detail_output_flag && write_detail(p, *j.rec, L, j.ng, top.data(), ctx, j.err)) return 1;
     j.t_dock = now_s() - t0;
     return 0;
 }

@@ -75,8 +75,16 @@ int Parameter::initialize(int argc, char **argv, std::string &err) {
                 lig_file = std::string(optarg) + "_l_u.pdb";
                 have_r = have_l = true;
                 break;
-            case 'o': out_file = optarg; have_o = true; break;
-            case 'O': std::printf("# -O: detail output is not produced by this build\n"); break;
+            case 'o': {  // src/parameter.cpp:81-92
+                out_file = optarg;
+                have_o = true;
+                const bool dot_out = out_file.size() > 4 && out_file.compare(out_file.size() - 4, 4, ".out") == 0;
+                const std::string stem = dot_out ? out_file.substr(0, out_file.size() - 4) : out_file;
+                detail_file = stem + ".detail";
+                csv_file = stem + ".csv";
+                break;
+            }
+            case 'O': detail_output_flag = 1; break;
             case 'N': num_output = std::atoi(optarg); std::printf("# Number of output = %d\n", num_output); break;
             case 't': num_sort = std::atoi(optarg); std::printf("# Set number of scores per one angle = %d\n", num_sort); break;
             case 'E': case 'i': break;  // calculation-time log: not produced
@@ -136,7 +144,12 @@ int Parameter::initialize(int argc, char **argv, std::string &err) {
     }
     if (!have_r) { err = "[ERROR] Receptor PDB file is not assignment!!"; return 1; }
     if (!have_l) { err = "[ERROR] Ligand PDB file is not assignment!!"; return 1; }
-    if (!have_o) out_file = basename_noext(rec_file) + "-" + basename_noext(lig_file) + ".out";
+    if (!have_o) {  // src/parameter.cpp:320-365
+        const std::string stem = basename_noext(rec_file) + "-" + basename_noext(lig_file);
+        out_file = stem + ".out";
+        detail_file = stem + ".detail";
+        csv_file = stem + ".csv";
+    }
     std::printf("# Output file = %s\n", out_file.c_str());
     if (num_sort < 1 || num_sort > 16) { err = "-t must be between 1 and 16 in this build"; return 1; }
     if (load_rotation_set(rotation_set, zangle, err)) return 1;

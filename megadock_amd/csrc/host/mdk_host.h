@@ -35,6 +35,8 @@ class Parameter {
 public:
     // defaults of src/parameter.cpp:266-308
     std::string rec_file, lig_file, out_file;
+    std::string detail_file, csv_file;  // -O outputs (src/parameter.cpp:81-92, :357-364)
+    int detail_output_flag = 0;
     int score_func = 3;
     float grid_width = 1.2f;
     int num_grid = 0, num_fft = 0, num_fft_flag = 0;
@@ -178,6 +180,12 @@ int run(int argc, char **argv, std::string &err);
 // The .out decoy file (src/docking.cpp:399-453) from the merged top list (num_output entries).
 int write_decoys(const Parameter &prm, const Protein &rec, const Protein &lig, int num_grid, const SortScore *top,
                  std::string &err);
+
+// The detailed output (-O): real-space score parts of the 7 poses around the input
+// position and of the num_output decoys, in the .detail and .csv files
+// (src/docking.cpp:456-898); the sums run on the device through mdk_score_detail.
+int write_detail(const Parameter &prm, const Receptor &rec, const Ligand &lig, int num_grid, const SortScore *top,
+                 mdk_ctx *ctx, std::string &err);
 
 // Rotations per device batch for an FFT size: 8 GiB of batch buffers, at most 128.
 int auto_batch(int nf, int ng, int t);

@@ -33,6 +33,11 @@ size_t topn_scratch_bytes(int n);
 hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *scratch, mdk_sortscore *out,
                        hipStream_t s);
 
+// real-space detail sums (mdk_detail.hip)
+size_t detail_scratch_doubles(int max_batch, int ng);
+hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
+                         double *scratch, double *out_dev, hipStream_t s);
+
 // off-grid scores (mdk_grad.hip)
 size_t grad_scratch_doubles(int max_batch);
 hipError_t launch_spec_eval(const float2 *Q, const float *t_dev, int B, int nf, int H, int Hp, double *part,
@@ -464,6 +469,9 @@ struct mdk_ctx {
     float *d_bt = nullptr;        // translations of a batch (max_batch * 3)
     float *d_gout = nullptr;      // results of a batch (max_batch * 7)
     double *d_gpart = nullptr;    // partial sums
+    float *d_rpsc = nullptr;      // receptor shape grid without the ACE term (mdk_score_detail), ng^3
+    int *d_btrans = nullptr;      // grid translations of a batch (max_batch * 3)
+    double *d_dpart = nullptr;    // detail sums: per (pose, slab) partials, then per pose
     bool have_recgrid = false;
     void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
     mdk_sortscore *d_topn_out = nullptr;
@@ -668,7 +676,8 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         dmalloc(&c->d_rc2, na) || dmalloc(&c->d_rs2, na) || dmalloc(&c->d_charge, na) || dmalloc(&c->d_gc, ng) ||
         dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) || dmalloc(&c->d_recgrid, ng3) ||
         dmalloc(&c->d_wshape, na) || dmalloc(&c->d_bt, 3 * B) || dmalloc(&c->d_gout, 7 * B) ||
-        dmalloc(&c->d_gpart, grad_scratch_doubles((int)B)))
+        dmalloc(&c->d_gpart, grad_scratch_doubles((int)B)) || dmalloc(&c->d_rpsc, ng3) || dmalloc(&c->d_btrans, 3 * B) ||
+        dmalloc(&c->d_dpart, detail_scratch_doubles((int)B, ng)))
         return bail(1);
     for (int k = 0; k < c->nslots; k++) {
         BatchSlot &S = c->slot[k];
@@ -727,7 +736,7 @@ int mdk_destroy(mdk_ctx *c) {
         if (c->slot[k].stream) (void)hipStreamSynchronize(c->slot[k].stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
                     c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out,
-                    c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart};
+                    c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart, c->d_rpsc, c->d_btrans, c->d_dpart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
@@ -1078,6 +1087,34 @@ int mdk_score_companion(mdk_ctx *c, const float *rotcoef, const float *t, int B,
                             c->stream));
     HIPCHK(hipMemcpyAsync(out, c->d_gout, sizeof(float) * 7 * B, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, double *out) {
+    if (!c || !rec_rpsc || !rotcoef || !trans || !out) return fail("mdk_score_detail: null argument");
+    if (!c->have_recgrid || !c->have_lig)
+        return fail("mdk_score_detail: receptor grids (mdk_set_receptor) and ligand must be set first");
+    if (B < 0) return fail("mdk_score_detail: negative pose count");
+    for (int q = 0; q < 3 * B; q++)
+        if (trans[q] < -c->ng || trans[q] >= 3 * c->ng) return fail("mdk_score_detail: translation out of range");
+    HIPCHK(hipSetDevice(c->device));
+    const int ng = c->ng;
+    const size_t ng3 = (size_t)ng * ng * ng;
+    BatchSlot &S = c->slot[0];
+    HIPCHK(hipMemcpyAsync(c->d_rpsc, rec_rpsc, sizeof(float) * ng3, hipMemcpyHostToDevice, c->stream));
+    double *sums = c->d_dpart + (size_t)c->max_batch * ng * 6;
+    for (int q0 = 0; q0 < B; q0 += c->max_batch) {  // batches of the context's size
+        const int nb = B - q0 < c->max_batch ? B - q0 : c->max_batch;
+        HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef + 9 * (size_t)q0, sizeof(float) * 9 * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_btrans, trans + 3 * (size_t)q0, sizeof(int) * 3 * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(launch_voxelize(c->lig, c->d_brot, nb, S.vs, S.d_vidx, S.d_chg, c->stream));
+        HIPCHK(launch_vox_expand(vox_grid(c, S), nb, S.d_lg, c->stream));
+        HIPCHK(launch_detail(S.d_lg, c->d_recgrid, c->d_rpsc, c->d_btrans, nb, ng, c->d_dpart, sums, c->stream));
+        HIPCHK(hipMemcpyAsync(out + 6 * (size_t)q0, sums, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));  // the staging buffers are reused by the next batch
+    }
     return 0;
 }
 

@@ -266,6 +266,32 @@ def correlation_fp64(nf, rec_grid_c, lig_grid_c):
     return np.real(np.fft.ifftn(np.conj(R) * L)) * nf ** 3
 
 
+def detail_sums(ng, rec_r, rec_i, rec_rpsc, lig_r, lig_i, trans):
+    """Real-space sums of Docking::output_detail (src/docking.cpp:596-657, :710-801) at
+    the grid translation `trans` (3 ints, a decoy's index[1..3]): per axis
+    s = (3 ng - t) mod 2 ng, ligand voxel i meets receptor voxel i + s - ng inside the
+    grid; products in float32 as the reference forms them, sums in float64.  Returns
+    (rPSC+, rPSC-, ELEC+, ELEC-, ACE+, ACE-) with rPSC = Rp Lr, ELEC = Ri Li and
+    ACE = (Rr - Rp) Lr (Rp: receptor shape grid built without the ACE term)."""
+    g = [np.asarray(a, np.float32).reshape(ng, ng, ng) for a in (rec_r, rec_i, rec_rpsc, lig_r, lig_i)]
+    rr, ri, rp, lr, li = g
+    lsl, rsl = [], []
+    for a in range(3):
+        s = (3 * ng - int(trans[a])) % (2 * ng)
+        lo, hi = (ng - s, ng) if s < ng else (0, 2 * ng - s)
+        lsl.append(slice(lo, hi))
+        rsl.append(slice(lo + s - ng, hi + s - ng))
+    lsl, rsl = tuple(lsl), tuple(rsl)
+    p = rp[rsl] * lr[lsl]
+    e = ri[rsl] * li[lsl]
+    c = (rr[rsl] - rp[rsl]) * lr[lsl]
+    out = []
+    for v in (p, e, c):
+        v = v.astype(np.float64)
+        out += [v[v > 0].sum(), v[v < 0].sum()]
+    return np.array(out)
+
+
 # ------------------------------------------------------------------ reference build
 def ref_load(rec_pdb, lig_pdb, extra=()):
     r = ref()

@@ -152,3 +152,35 @@ def test_dock_loop_equals_stepwise(golden_dir):
         out = O.fft3d(nf, O.convolution(rr, ri, L), +1)
         exp = O.score_sort(nf, 2, out, int(a))
         assert np.array_equal(got[2 * q:2 * q + 2].view(np.uint8), exp.view(np.uint8))
+
+
+@pytest.mark.parametrize("ng", [6, 7])
+def test_detail_sums_are_the_correlation_score(ng):
+    """The real-space sums of the detailed output add up to the docking score at the same
+    translation (rPSC + ACE = Rr Lr and ELEC = Ri Li over the overlap), which the FFT
+    correlation of the padded grids gives independently, for translations that wrap and
+    for the edge values -1 and nf a neighbour of the correct position can take."""
+    rng = np.random.default_rng(ng)
+    nf = 2 * ng
+    rr, ri, rp, lr, li = [rng.standard_normal(ng ** 3).astype(np.float32) for _ in range(5)]
+    # the unnormalised inverse transform: nf^3 times the real-space correlation
+    smap = O.correlation_fp64(nf, O.embed(ng, nf, rr, ri), O.embed(ng, nf, lr, li)).reshape(nf, nf, nf)
+    for t in [(0, 0, 0), (1, 2, 3), (nf - 1, ng, 5), (ng // 2, nf - 2, ng + 1), (3, 4, nf - 1)]:
+        d = O.detail_sums(ng, rr, ri, rp, lr, li, t)
+        total = d.sum() * nf ** 3
+        assert abs(total - smap[t]) <= 1e-5 * np.abs(smap).max(), (t, total, smap[t])
+        assert d[0] >= 0 and d[2] >= 0 and d[4] >= 0 and d[1] <= 0 and d[3] <= 0 and d[5] <= 0
+    for t, same in [((-1, 0, 0), (nf - 1, 0, 0)), ((nf, 3, 1), (0, 3, 1))]:
+        assert np.array_equal(O.detail_sums(ng, rr, ri, rp, lr, li, t), O.detail_sums(ng, rr, ri, rp, lr, li, same))
+
+
+def test_detail_sums_split_and_ace_part():
+    """Constant grids: every overlap voxel contributes the same products, so the sums are
+    the overlap count times them, with the signs going to the right halves."""
+    ng = 8
+    one = np.ones(ng ** 3, np.float32)
+    d = O.detail_sums(ng, 3 * one, -2 * one, one, 2 * one, 0.5 * one, (1, 0, 3))
+    cnt = (ng - 1) * ng * (ng - 3)  # a shift by t voxels overlaps ng - t planes
+    assert d[0] == 2.0 * cnt and d[1] == 0.0        # rPSC = 1 * 2
+    assert d[2] == 0.0 and d[3] == -1.0 * cnt       # ELEC = -2 * 0.5
+    assert d[4] == 4.0 * cnt and d[5] == 0.0        # ACE = (3 - 1) * 2
