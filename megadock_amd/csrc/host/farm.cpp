@@ -201,7 +201,7 @@ public:
             std::vector<float> xyz(3 * (size_t)rec.num_atoms());
             rec.shift_center(xyz.data());
             rec.voxelize(ng, p.grid_width, grid_coord(ng, p.grid_width), xyz.data(), p.ace_ratio, p.rec_core,
-                         p.elec_ratio, gr->r.data(), gr->i.data());
+                         p.elec_ratio, gr->r.data(), gr->i.data(), 1);  // the preparation threads run in parallel
             e->grids = gr;
         });
         return e->grids;

@@ -108,8 +108,11 @@ public:
     explicit Receptor(const std::string &file) : Protein(file) {}
     // Shape (rPSC + ACE) and electrostatics grids, ng^3 each, from centred coordinates
     // (src/receptor.cpp:20-372).
+    // The per-voxel sums run over x-planes on `threads` host threads (0: the machine's
+    // cores, at most 16), each voxel's terms in atom order, so the grids do not depend on it.
     void voxelize(int ng, float gw, const std::vector<float> &grid_coord, const float *xyz,
-                  float aceratio, float rec_core, float eratio, float *grid_r, float *grid_i) const;
+                  float aceratio, float rec_core, float eratio, float *grid_r, float *grid_i,
+                  int threads = 0) const;
 };
 
 class Ligand : public Protein {

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "mdk_host.h"
 
@@ -36,10 +37,24 @@ Box atom_box(const float *p, float gw, int ng, int range) {
     return b;
 }
 
+// fn(i0, i1) over x-planes [i0, i1) split among up to `threads` threads.  Every voxel
+// belongs to one plane, so per-voxel sums keep their atom order whatever the split.
+template <class F> void over_planes(int ng, int threads, F &&fn) {
+    if (threads <= 0) threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    threads = std::min(threads, ng);
+    if (threads <= 1) {
+        fn(0, ng);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) pool.emplace_back([&, t] { fn(ng * t / threads, ng * (t + 1) / threads); });
+    for (auto &th : pool) th.join();
+}
+
 }  // namespace
 
 void Receptor::voxelize(int ng, float gw, const std::vector<float> &gc, const float *xyz, float aceratio,
-                        float rec_core, float eratio, float *grid_r, float *grid_i) const {
+                        float rec_core, float eratio, float *grid_r, float *grid_i, int threads) const {
     const int na = num_atoms();
     const size_t n1 = ng, n2 = n1 * n1, n3 = n2 * n1;
     const float eps = rec_core, open = -7777.0f;
@@ -59,19 +74,21 @@ void Receptor::voxelize(int ng, float gw, const std::vector<float> &gc, const fl
 
     // core fill
     const int core_range = (int)((2.4 + (double)gw - 0.01) / (double)gw);
-    for (int l = 0; l < na; l++) {
-        const float r = radius[l], rc2 = r * r * 1.5f;
-        const Box b = atom_box(&xyz[3 * l], gw, ng, core_range);
-        const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
-        for (int i = b.lo[0]; i < b.hi[0]; i++) {
-            if (ax[i] > rc2) continue;
-            for (int j = b.lo[1]; j < b.hi[1]; j++) {
-                if (ax[i] + ay[j] > rc2) continue;
-                for (int k = b.lo[2]; k < b.hi[2]; k++)
-                    if (ax[i] + ay[j] + az[k] < rc2) at(i, j, k) = eps;
+    over_planes(ng, threads, [&](int i0, int i1) {
+        for (int l = 0; l < na; l++) {
+            const float r = radius[l], rc2 = r * r * 1.5f;
+            const Box b = atom_box(&xyz[3 * l], gw, ng, core_range);
+            const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
+            for (int i = std::max(b.lo[0], i0); i < std::min(b.hi[0], i1); i++) {
+                if (ax[i] > rc2) continue;
+                for (int j = b.lo[1]; j < b.hi[1]; j++) {
+                    if (ax[i] + ay[j] > rc2) continue;
+                    for (int k = b.lo[2]; k < b.hi[2]; k++)
+                        if (ax[i] + ay[j] + az[k] < rc2) at(i, j, k) = eps;
+                }
             }
         }
-    }
+    });
 
     // open space: rays from each face until the first core voxel
     for (size_t p = 0; p < n2; p++) {
@@ -110,23 +127,25 @@ void Receptor::voxelize(int ng, float gw, const std::vector<float> &gc, const fl
     // surface score: rPSC + receptor ACE
     std::vector<float> surf(n3, 0.0f);
     const int surf_range = (int)((12 + (double)gw - 0.01) / (double)gw);
-    for (int l = 0; l < na; l++) {
-        const float rj = (radius[l] + 3.6f) * (radius[l] + 3.6f);
-        const double add = 1.0 + (double)ace[l] * (-0.8) * (double)aceratio;
-        const Box b = atom_box(&xyz[3 * l], gw, ng, surf_range);
-        const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
-        for (int i = b.lo[0]; i < b.hi[0]; i++) {
-            if (ax[i] > rj) continue;
-            for (int j = b.lo[1]; j < b.hi[1]; j++) {
-                if (ax[i] + ay[j] > rj) continue;
-                for (int k = b.lo[2]; k < b.hi[2]; k++)
-                    if (ax[i] + ay[j] + az[k] < rj) {
-                        float &s = surf[i * n2 + j * n1 + k];
-                        s = (float)((double)s + add);
-                    }
+    over_planes(ng, threads, [&](int i0, int i1) {
+        for (int l = 0; l < na; l++) {
+            const float rj = (radius[l] + 3.6f) * (radius[l] + 3.6f);
+            const double add = 1.0 + (double)ace[l] * (-0.8) * (double)aceratio;
+            const Box b = atom_box(&xyz[3 * l], gw, ng, surf_range);
+            const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
+            for (int i = std::max(b.lo[0], i0); i < std::min(b.hi[0], i1); i++) {
+                if (ax[i] > rj) continue;
+                for (int j = b.lo[1]; j < b.hi[1]; j++) {
+                    if (ax[i] + ay[j] > rj) continue;
+                    for (int k = b.lo[2]; k < b.hi[2]; k++)
+                        if (ax[i] + ay[j] + az[k] < rj) {
+                            float &s = surf[i * n2 + j * n1 + k];
+                            s = (float)((double)s + add);
+                        }
+                }
             }
         }
-    }
+    });
     for (size_t q = 0; q < n3; q++)
         if (grid_r[q] != eps) grid_r[q] = surf[q];
 
@@ -134,24 +153,26 @@ void Receptor::voxelize(int ng, float gw, const std::vector<float> &gc, const fl
     const float beta = -2800.0f;
     const float gwl = gc[1] - gc[0];
     const float charge_per_grid = -1 * gwl * 80.0f / (beta * eratio);
-    for (int l = 0; l < na; l++) {
-        const float q = charge[l], aq = std::fabs(q);
-        if (aq <= 1.0e-6f) continue;
-        const int range = (double)aq < 0.07 ? 4 : ((double)aq < 0.21 ? 5 : (int)(aq / charge_per_grid));
-        const Box b = atom_box(&xyz[3 * l], gwl, ng, range);
-        const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
-        for (int i = b.lo[0]; i < b.hi[0]; i++)
-            for (int j = b.lo[1]; j < b.hi[1]; j++) {
-                const float dij = ax[i] + ay[j];
-                for (int k = b.lo[2]; k < b.hi[2]; k++) {
-                    const size_t w = i * n2 + j * n1 + k;
-                    if (grid_r[w] < 0.0) continue;
-                    const float d = std::sqrt(dij + az[k]);
-                    const float er = d >= 8.0f ? 80.0f : (d > 6.0f ? 38.0f * d + -224.0f : 4.0f);
-                    grid_i[w] += eratio * beta * q / er / d;
+    over_planes(ng, threads, [&](int i0, int i1) {
+        for (int l = 0; l < na; l++) {
+            const float q = charge[l], aq = std::fabs(q);
+            if (aq <= 1.0e-6f) continue;
+            const int range = (double)aq < 0.07 ? 4 : ((double)aq < 0.21 ? 5 : (int)(aq / charge_per_grid));
+            const Box b = atom_box(&xyz[3 * l], gwl, ng, range);
+            const float *ax = &d2x[l * n1], *ay = &d2y[l * n1], *az = &d2z[l * n1];
+            for (int i = std::max(b.lo[0], i0); i < std::min(b.hi[0], i1); i++)
+                for (int j = b.lo[1]; j < b.hi[1]; j++) {
+                    const float dij = ax[i] + ay[j];
+                    for (int k = b.lo[2]; k < b.hi[2]; k++) {
+                        const size_t w = i * n2 + j * n1 + k;
+                        if (grid_r[w] < 0.0) continue;
+                        const float d = std::sqrt(dij + az[k]);
+                        const float er = d >= 8.0f ? 80.0f : (d > 6.0f ? 38.0f * d + -224.0f : 4.0f);
+                        grid_i[w] += eratio * beta * q / er / d;
+                    }
                 }
-            }
-    }
+        }
+    });
 }
 
 }  // namespace mdk
