@@ -134,8 +134,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the rotation loop runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU: the rotation loop runs on a single MI355X here; rotations "
+              "would shard as contiguous blocks per GPU with one merge of the per-GPU top-N, which "
+              "this build does not run", file=sys.stderr)
         sys.exit(2)
 
     import megadock_amd as M

@@ -16,7 +16,8 @@ Two refinements:
   refine_hard()   ascent whose steps are accepted only when the hard-voxel score rises:
                   translation along the exact gradient of the band-limited hard score
                   (SpectralScore), rotation along the companion's torque (the hard
-                  voxelizer has no rotation gradient), so the judge's score never drops.
+                  voxelizer has no rotation gradient), so the hard-voxel score, by which
+                  refined poses are judged, never drops.
 """
 import numpy as np
 import torch
