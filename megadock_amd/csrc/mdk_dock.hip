@@ -460,7 +460,7 @@ struct mdk_ctx {
     // batch buffers: one set per stream; mdk_dock_range alternates batches between the
     // streams so that one batch's memory-bound passes overlap another's compute-bound ones
     BatchSlot slot[2];
-    int nslots = 1;                // MDK_STREAMS (1 or 2)
+    int nslots = 1;                // batch slots and streams (MDK_STREAMS: 1 or 2)
     bool use_graph = false;        // MDK_GRAPH: full batches of mdk_dock_range replay a captured graph
     mdk_sortscore *d_results = nullptr; // nrot * t
     // off-grid scores and gradients (mdk_grad.hip)
@@ -661,8 +661,11 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     };
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail("mdk_create: hipSetDevice failed"); }
     {
+        // two batches in flight on two streams (one kernel's tail overlaps the next batch's
+        // first kernels; 61.0k against 59.6k rotations/s at 144^3), unless the batches are
+        // small or MDK_STREAMS asks for 1
         const char *ns = std::getenv("MDK_STREAMS");
-        c->nslots = (ns && std::atoi(ns) == 2) ? 2 : 1;  // two streams measured no faster at 144^3
+        c->nslots = ns ? (std::atoi(ns) == 2 ? 2 : 1) : (max_batch >= 32 ? 2 : 1);
         const char *gr = std::getenv("MDK_GRAPH");
         c->use_graph = gr && gr[0] == '1';
     }
