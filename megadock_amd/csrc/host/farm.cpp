@@ -30,6 +30,7 @@
 #include <memory>
 #include <mutex>
 #include <sstream>
+#include <stdexcept>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -474,7 +475,13 @@ int run_table(int argc, char **argv, std::string &err) {
             std::unique_ptr<Job> j;
             while (todo.pop(j)) {
                 j->argv = expand_job(o, t, j->line, j->attempt);
-                if (prepare(*j, cache)) {
+                int rc = 1;
+                try {
+                    rc = prepare(*j, cache);
+                } catch (const std::exception &e) {  // e.g. out of host memory: this job fails
+                    j->err = std::string("preparation: ") + e.what();
+                }
+                if (rc) {
                     j->t_prep = 0;
                     finish(std::move(j), false);
                 } else {
@@ -488,7 +495,12 @@ int run_table(int argc, char **argv, std::string &err) {
         Contexts cs;
         std::unique_ptr<Job> j;
         while (ready.pop(j)) {
-            const bool ok = dock(*j, cs) == 0;
+            bool ok = false;
+            try {
+                ok = dock(*j, cs) == 0;
+            } catch (const std::exception &e) {
+                j->err = std::string("docking: ") + e.what();
+            }
             std::printf("# job %05d (%s): %s\n", j->id, j->line.c_str(), ok ? "done" : j->err.c_str());
             std::fflush(stdout);
             finish(std::move(j), ok);
