@@ -59,7 +59,9 @@ static std::string basename_noext(std::string f) {  // src/parameter.cpp:320-352
 int Parameter::initialize(int argc, char **argv, std::string &err) {
     bool have_r = false, have_l = false, have_o = false;
     int f1 = 0, f2 = 0;
-    optind = 1;
+    // optind = 0 makes glibc's getopt start afresh: with 1 it may resume inside the last
+    // argument of an earlier parse (an argument vector that may no longer exist)
+    optind = 0;
     int ch;
     while ((ch = getopt(argc, argv, "R:L:B:U:o:ON:t:E:a:b:e:d:F:v:Dhy:z:r:f:i:jkl:S:G:T:W:")) != -1) {
         switch (ch) {

@@ -1,5 +1,5 @@
 """GPU: the job-table farm (mdkh_run_table) docks every job of a table exactly as the
-single-pair program does (byte-identical .out files), with one context per FFT size
+single-pair program does (byte-identical .out, .detail and .csv files), with one context per FFT size
 reused across jobs (ligands of fewer atoms than the context was made for), and retries
 a failing job before it reports it."""
 import os
@@ -27,7 +27,7 @@ def test_farm_matches_single_runs(tmp_path, golden_dir):
     pairs = [(r, r), (r, l), (l, r), (l, l)] + [W.make_c2_pair(a, b, str(wl)) for a, b in [(1, 0), (4, 5)]]
     lines = ["%s\t%s\t%s" % (a, b, tmp_path / ("farm_%d" % k)) for k, (a, b) in enumerate(pairs)]
     tb = tmp_path / "jobs.table"
-    tb.write_text("TITLE=farm test\nPARAM=-R $1 -L $2 -o $3.out -N 100 -t 2\n" + "\n".join(lines) + "\n")
+    tb.write_text("TITLE=farm test\nPARAM=-R $1 -L $2 -o $3.out -N 100 -t 2 -O\n" + "\n".join(lines) + "\n")
     log = tmp_path / "master.log"
     print("farm of %d jobs" % len(pairs), flush=True)
     M.run_table(["-tb", str(tb), "-lg", str(log), "-pj", "4"])
@@ -35,8 +35,10 @@ def test_farm_matches_single_runs(tmp_path, golden_dir):
     assert all(rows["%05d" % (k + 1)][2] == "done" for k in range(len(pairs)))
     for k, (a, b) in enumerate(pairs):
         single = tmp_path / ("single_%d.out" % k)
-        M.run(["-R", a, "-L", b, "-o", str(single), "-N", "100", "-t", "2"])
-        assert (tmp_path / ("farm_%d.out" % k)).read_bytes() == single.read_bytes(), (k, a, b)
+        M.run(["-R", a, "-L", b, "-o", str(single), "-N", "100", "-t", "2", "-O"])
+        for ext in (".out", ".detail", ".csv"):  # the decoys and the detailed output (-O)
+            f, g = tmp_path / ("farm_%d%s" % (k, ext)), tmp_path / ("single_%d%s" % (k, ext))
+            assert f.read_bytes() == g.read_bytes(), (k, ext)
 
 
 def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):

@@ -67,3 +67,12 @@ def test_specialised_size_lists_match_the_build():
     listed = [int(x) for x in re.findall(r"X\((\d+)\)", src)]
     assert listed == M.FURTHER_NF
     assert all("case %d: return run_pipe<%d, true>" % (n, n) in src for n in M.TUNED_NF)
+
+
+def test_command_line_parses_afresh_after_a_trailing_flag(tmp_path):
+    """A parse that ends on a flag (-O) must not leak into the next one: the second call
+    reads its own arguments (here it reaches the GPU check, or fails on the missing PDB)."""
+    for args in (["-R", "r.pdb", "-L", "l.pdb", "-O"], ["-R", str(tmp_path / "none_r.pdb"), "-L", "l.pdb", "-t", "2"]):
+        with pytest.raises(M.MdkError) as e:
+            M.run(args)
+        assert "invalid option" not in str(e.value) and "Usage" not in str(e.value), str(e.value)
