@@ -1065,7 +1065,8 @@ int mdk_score_spectral(mdk_ctx *c, const float *rotcoef, const float *t, int B, 
     if (!c || !rotcoef || !t || !out) return fail("mdk_score_spectral: null argument");
     if (!c->have_rec || !c->have_lig) return fail("mdk_score_spectral: receptor and ligand must be set first");
     if (B < 1 || B > c->max_batch) return fail("mdk_score_spectral: batch size out of range");
-    if (!c->use_pipe) return fail("mdk_score_spectral: needs the specialised kernels (nf = 80, 96, 144 or 256)");
+    if (!c->use_pipe)
+        return fail("mdk_score_spectral: needs an FFT size with specialised kernels (and MDK_FORCE_GENERIC unset)");
     HIPCHK(hipSetDevice(c->device));
     BatchSlot &S = c->slot[0];
     HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));

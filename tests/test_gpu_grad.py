@@ -32,8 +32,13 @@ def _ctx(lig, ng, gr, gi, t=1, max_batch=8):
     return ctx
 
 
-def test_spectral_score_and_gradient_vs_fp64(pair):
+@pytest.mark.parametrize("ngs", [None, 50, 63])
+def test_spectral_score_and_gradient_vs_fp64(pair, ngs):
+    """nf = 96 (tuned kernels) and 100, 126 (further sizes, default variants)."""
     rec, lig, ng, gr, gi = pair
+    if ngs is not None:
+        ng = ngs
+        gr, gi = O.receptor_voxel(ng, 1.2, rec)
     nf = 2 * ng
     ctx = _ctx(lig, ng, gr, gi)
     ang = O.load_rotations(3600)
