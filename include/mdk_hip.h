@@ -60,6 +60,20 @@ int mdk_set_receptor(mdk_ctx *ctx, const float *grid_r, const float *grid_i);
  * arrays of src/fft_process.cpp:257-260). */
 int mdk_set_receptor_spectrum(mdk_ctx *ctx, const float *re, const float *im);
 
+/* Receptor grids computed on the device from its atoms, bit-exact with the reference's
+ * Receptor::rpscace + Receptor::electro (src/receptor.cpp:20-346): xyz (na*3) centred as
+ * Protein::shift_center leaves them (src/protein.cpp:114-131), radius, charge and ACE
+ * per atom (na), grid coordinates (ng, src/docking.cpp:139-141), voxel pitch gw, ACE
+ * ratio (-d), core value (-a) and electrostatics ratio (-e); then transformed as by
+ * mdk_set_receptor.  Replaces the host part of Docking::rec_init (src/docking.cpp:128-159). */
+int mdk_set_receptor_atoms(mdk_ctx *ctx, int na, const float *xyz, const float *radius, const float *charge,
+                           const float *ace, const float *grid_coord, float gw, float aceratio, float rec_core,
+                           float eratio);
+
+/* Read the receptor grids back (ng^3 each): those of mdk_set_receptor or
+ * mdk_set_receptor_atoms. */
+int mdk_get_receptor_grids(mdk_ctx *ctx, float *grid_r, float *grid_i);
+
 /* Read the receptor spectrum back (SoA, nf^3 each). */
 int mdk_get_receptor_spectrum(mdk_ctx *ctx, float *re, float *im);
 

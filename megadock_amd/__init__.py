@@ -22,7 +22,7 @@ ABI_SYMBOLS = [
     "mdk_set_ligand", "mdk_dock", "mdk_set_rotations", "mdk_dock_range", "mdk_sync",
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
-    "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail",
+    "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail", "mdk_set_receptor_atoms", "mdk_get_receptor_grids",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -112,6 +112,24 @@ class Context:
         gi = np.ascontiguousarray(grid_i, np.float32)
         assert gr.size == self.ng ** 3 and gi.size == self.ng ** 3
         _check(lib().mdk_set_receptor(self.h, _fp(gr), _fp(gi)), "mdk_set_receptor")
+
+    def set_receptor_atoms(self, xyz_centred, radius, charge, ace, grid_coord, gw=1.2, aceratio=1.0,
+                           rec_core=-45.0, eratio=1.0):
+        """Receptor grids built on the device from the atoms (centred coordinates), then
+        transformed; bit-exact with the host producer (receptor_grids)."""
+        xyz = np.ascontiguousarray(xyz_centred, np.float32).ravel()
+        na = xyz.size // 3
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (radius, charge, ace, grid_coord)]
+        _check(lib().mdk_set_receptor_atoms(self.h, C.c_int(na), _fp(xyz), *[_fp(a) for a in arrs], C.c_float(gw),
+                                            C.c_float(aceratio), C.c_float(rec_core), C.c_float(eratio)),
+               "mdk_set_receptor_atoms")
+
+    def receptor_grids(self):
+        n3 = self.ng ** 3
+        gr = np.zeros(n3, np.float32)
+        gi = np.zeros(n3, np.float32)
+        _check(lib().mdk_get_receptor_grids(self.h, _fp(gr), _fp(gi)), "mdk_get_receptor_grids")
+        return gr, gi
 
     def set_receptor_spectrum(self, re, im):
         re = np.ascontiguousarray(re, np.float32)

@@ -111,17 +111,20 @@ int Docking::initialize(int device, std::string &err) {
     return fft_->alloc_array(2 * num_grid_, device, err);
 }
 
+// The receptor grids are built on the device (mdk_set_receptor_atoms, bit-exact with
+// Receptor::voxelize) and transformed there.
 int Docking::rec_init(std::string &err) {
     const double t0 = now();
     const int ng = num_grid_;
     grid_coord_ = grid_coord(ng, prm_->grid_width);
     std::vector<float> xyz(3 * (size_t)rec_->num_atoms());
     rec_->shift_center(xyz.data());
-    const size_t ng3 = (size_t)ng * ng * ng;
-    std::vector<float> gr(ng3), gi(ng3);
-    rec_->voxelize(ng, prm_->grid_width, grid_coord_, xyz.data(), prm_->ace_ratio, prm_->rec_core,
-                   prm_->elec_ratio, gr.data(), gi.data());
-    if (fft_->receptor_fft(gr.data(), gi.data(), err)) return 1;
+    if (mdk_set_receptor_atoms(fft_->ctx(), rec_->num_atoms(), xyz.data(), rec_->radius.data(), rec_->charge.data(),
+                               rec_->ace.data(), grid_coord_.data(), prm_->grid_width, prm_->ace_ratio, prm_->rec_core,
+                               prm_->elec_ratio)) {
+        err = abi_error("mdk_set_receptor_atoms");
+        return 1;
+    }
     t_receptor = now() - t0;
     return 0;
 }

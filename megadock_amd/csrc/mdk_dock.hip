@@ -38,6 +38,12 @@ size_t detail_scratch_doubles(int max_batch, int ng);
 hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
                          double *scratch, double *out_dev, hipStream_t s);
 
+// receptor grids on the device (mdk_recvox.hip)
+size_t receptor_voxel_scratch_bytes(int ng);
+hipError_t launch_receptor_voxel(const float *xyz, const float *radius, const float *charge, const float *ace,
+                                 const float *gc, int na, int ng, float gw, float aceratio, float rec_core, float eratio,
+                                 float gc0, float gc1, void *scratch, float2 *out, hipStream_t s);
+
 // off-grid scores (mdk_grad.hip)
 size_t grad_scratch_doubles(int max_batch);
 hipError_t launch_spec_eval(const float2 *Q, const float *t_dev, int B, int nf, int H, int Hp, double *part,
@@ -757,15 +763,11 @@ int mdk_destroy(mdk_ctx *c) {
     return 0;
 }
 
-int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
-    if (!c || !grid_r || !grid_i) return fail("mdk_set_receptor: null argument");
-    HIPCHK(hipSetDevice(c->device));
+// receptor spectrum from the receptor grids already in d_recgrid (src/fft_process.cpp:212-303)
+int receptor_transform(mdk_ctx *c) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     const size_t ng3 = (size_t)ng * ng * ng;
-    std::vector<float2> h(ng3);
-    for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
-    HIPCHK(hipMemcpyAsync(c->slot[0].d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_recgrid, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->slot[0].d_lg, c->d_recgrid, sizeof(float2) * ng3, hipMemcpyDeviceToDevice, c->stream));
     const int lines_z = ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), c->stream>>>(c->slot[0].d_lg, c->slot[0].d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a1, c->slot[0].d_a2, c->plan, c->d_tw, ng, nd);
@@ -774,6 +776,71 @@ int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_rec = true;
     c->have_recgrid = true;
+    return 0;
+}
+
+int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
+    if (!c || !grid_r || !grid_i) return fail("mdk_set_receptor: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
+    std::vector<float2> h(ng3);
+    for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
+    HIPCHK(hipMemcpyAsync(c->d_recgrid, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    return receptor_transform(c);
+}
+
+int mdk_set_receptor_atoms(mdk_ctx *c, int na, const float *xyz, const float *radius, const float *charge,
+                           const float *ace, const float *grid_coord, float gw, float aceratio, float rec_core,
+                           float eratio) {
+    if (!c || na < 0 || (na > 0 && (!xyz || !radius || !charge || !ace)) || !grid_coord)
+        return fail("mdk_set_receptor_atoms: invalid argument");
+    if (!(gw > 0.0f) || c->ng < 2) return fail("mdk_set_receptor_atoms: invalid grid");
+    HIPCHK(hipSetDevice(c->device));
+    const int ng = c->ng;
+    const size_t n = na > 0 ? (size_t)na : 1;
+    float *d_xyz = nullptr, *d_rad = nullptr, *d_chg = nullptr, *d_ace = nullptr, *d_gc = nullptr;
+    void *scratch = nullptr;
+    int rc = 0;
+    if (hipMalloc(&d_xyz, sizeof(float) * 3 * n) != hipSuccess || hipMalloc(&d_rad, sizeof(float) * n) != hipSuccess ||
+        hipMalloc(&d_chg, sizeof(float) * n) != hipSuccess || hipMalloc(&d_ace, sizeof(float) * n) != hipSuccess ||
+        hipMalloc(&d_gc, sizeof(float) * ng) != hipSuccess ||
+        hipMalloc(&scratch, receptor_voxel_scratch_bytes(ng)) != hipSuccess)
+        rc = fail("mdk_set_receptor_atoms: out of device memory");
+    if (!rc && na > 0 &&
+        (hipMemcpy(d_xyz, xyz, sizeof(float) * 3 * na, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(d_rad, radius, sizeof(float) * na, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(d_chg, charge, sizeof(float) * na, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(d_ace, ace, sizeof(float) * na, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail("mdk_set_receptor_atoms: upload failed");
+    if (!rc && hipMemcpy(d_gc, grid_coord, sizeof(float) * ng, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail("mdk_set_receptor_atoms: upload failed");
+    if (!rc && launch_receptor_voxel(d_xyz, d_rad, d_chg, d_ace, d_gc, na, ng, gw, aceratio, rec_core, eratio,
+                                     grid_coord[0], ng > 1 ? grid_coord[1] : grid_coord[0], scratch, c->d_recgrid,
+                                     c->stream) != hipSuccess)
+        rc = fail("mdk_set_receptor_atoms: kernel launch failed");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail("mdk_set_receptor_atoms: voxelization failed");
+    (void)hipFree(d_xyz);
+    (void)hipFree(d_rad);
+    (void)hipFree(d_chg);
+    (void)hipFree(d_ace);
+    (void)hipFree(d_gc);
+    (void)hipFree(scratch);
+    if (rc) return rc;
+    return receptor_transform(c);
+}
+
+int mdk_get_receptor_grids(mdk_ctx *c, float *grid_r, float *grid_i) {
+    if (!c || !grid_r || !grid_i) return fail("mdk_get_receptor_grids: null argument");
+    if (!c->have_recgrid) return fail("mdk_get_receptor_grids: no receptor grids set");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
+    std::vector<float2> h(ng3);
+    HIPCHK(hipMemcpyAsync(h.data(), c->d_recgrid, sizeof(float2) * ng3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < ng3; i++) {
+        grid_r[i] = h[i].x;
+        grid_i[i] = h[i].y;
+    }
     return 0;
 }
 
