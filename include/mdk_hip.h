@@ -70,6 +70,16 @@ int mdk_set_receptor_atoms(mdk_ctx *ctx, int na, const float *xyz, const float *
                            const float *ace, const float *grid_coord, float gw, float aceratio, float rec_core,
                            float eratio);
 
+/* Decoy coordinates of decoygen (src/decoygen.cpp:65-103) for nd decoys at once, on the
+ * device: xyz (na*3) the ligand atoms minus the ligand centre of the .out file, rot_pre
+ * (9) the coefficients of the .out file's second-line angles, rotcoef (nd*9) those of the
+ * decoys' angles (both as mdk_rotation_coef forms them, which is rotateAtom's
+ * expression), trans (nd*3) the decoys' grid translations already mapped to t - nf for
+ * t >= nf/2, spacing the grid width and offset (3) the receptor centre;
+ * out (nd*na*3) = R_decoy (R_pre xyz) - trans * spacing + offset, in float as decoygen. */
+int mdk_decoy_coords(int device, const float *xyz, int na, const float *rot_pre, const float *rotcoef, const int *trans,
+                     int nd, float spacing, const float *offset, float *out);
+
 /* Read the receptor grids back (ng^3 each): those of mdk_set_receptor or
  * mdk_set_receptor_atoms. */
 int mdk_get_receptor_grids(mdk_ctx *ctx, float *grid_r, float *grid_i);

@@ -78,3 +78,36 @@ hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *r
     k_detail_finish<<<B, 64, 0, s>>>(scratch, ng, out_dev);
     return hipGetLastError();
 }
+
+// ------------------------------------------------------------------ decoy coordinates
+// Atom a of decoy d (decoygen's createPDB, src/decoygen.cpp:65-103): the ligand atom
+// relative to its centre, rotated by the .out file's receptor-line angles and then by the
+// decoy's angles (rotation coefficients from the host, as decoygen's rotateAtom forms
+// them), moved by minus its grid translation times the spacing and by the receptor centre.
+namespace {
+__global__ void k_decoy_coords(const float *__restrict__ xyz, int na, const float *__restrict__ pre,
+                               const float *__restrict__ rot, const int *__restrict__ trans, float spacing, float r1,
+                               float r2, float r3, float *__restrict__ out) {
+    const int nb = (na + 255) / 256, d = blockIdx.x / nb, a = (blockIdx.x - d * nb) * blockDim.x + threadIdx.x;
+    if (a >= na) return;
+    const float x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2];
+    const float x1 = pre[0] * x + pre[1] * y + pre[2] * z;
+    const float y1 = pre[3] * x + pre[4] * y + pre[5] * z;
+    const float z1 = pre[6] * x + pre[7] * y + pre[8] * z;
+    const float *R = rot + 9 * (size_t)d;
+    const float x2 = R[0] * x1 + R[1] * y1 + R[2] * z1;
+    const float y2 = R[3] * x1 + R[4] * y1 + R[5] * z1;
+    const float z2 = R[6] * x1 + R[7] * y1 + R[8] * z1;
+    float *o = out + 3 * ((size_t)d * na + a);
+    o[0] = x2 - trans[3 * d] * spacing + r1;
+    o[1] = y2 - trans[3 * d + 1] * spacing + r2;
+    o[2] = z2 - trans[3 * d + 2] * spacing + r3;
+}
+}  // namespace
+
+hipError_t launch_decoy_coords(const float *xyz, int na, const float *pre, const float *rot, const int *trans, int nd,
+                               float spacing, const float *r, float *out, hipStream_t s) {
+    const unsigned blocks = (unsigned)((na + 255) / 256) * (unsigned)nd;  // decoy-major, 256 atoms a block
+    k_decoy_coords<<<blocks, 256, 0, s>>>(xyz, na, pre, rot, trans, spacing, r[0], r[1], r[2], out);
+    return hipGetLastError();
+}

@@ -38,6 +38,9 @@ size_t detail_scratch_doubles(int max_batch, int ng);
 hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
                          double *scratch, double *out_dev, hipStream_t s);
 
+hipError_t launch_decoy_coords(const float *xyz, int na, const float *pre, const float *rot, const int *trans, int nd,
+                               float spacing, const float *r, float *out, hipStream_t s);
+
 // receptor grids on the device (mdk_recvox.hip)
 size_t receptor_voxel_scratch_bytes(int ng);
 hipError_t launch_receptor_voxel(const float *xyz, const float *radius, const float *charge, const float *ace,
@@ -1187,6 +1190,37 @@ int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, co
         HIPCHK(hipStreamSynchronize(c->stream));  // the staging buffers are reused by the next batch
     }
     return 0;
+}
+
+int mdk_decoy_coords(int device, const float *xyz, int na, const float *rot_pre, const float *rotcoef, const int *trans,
+                     int nd, float spacing, const float *offset, float *out) {
+    if (!xyz || !rot_pre || !rotcoef || !trans || !offset || !out || na < 1 || nd < 0)
+        return fail("mdk_decoy_coords: invalid argument");
+    if (nd == 0) return 0;
+    HIPCHK(hipSetDevice(device));
+    float *d_xyz = nullptr, *d_pre = nullptr, *d_rot = nullptr, *d_out = nullptr;
+    int *d_tr = nullptr;
+    int rc = 0;
+    const size_t no = 3 * (size_t)na * nd;
+    if (hipMalloc(&d_xyz, sizeof(float) * 3 * na) != hipSuccess || hipMalloc(&d_pre, sizeof(float) * 9) != hipSuccess ||
+        hipMalloc(&d_rot, sizeof(float) * 9 * nd) != hipSuccess || hipMalloc(&d_tr, sizeof(int) * 3 * nd) != hipSuccess ||
+        hipMalloc(&d_out, sizeof(float) * no) != hipSuccess)
+        rc = fail("mdk_decoy_coords: out of device memory");
+    if (!rc && (hipMemcpy(d_xyz, xyz, sizeof(float) * 3 * na, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_pre, rot_pre, sizeof(float) * 9, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_rot, rotcoef, sizeof(float) * 9 * nd, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_tr, trans, sizeof(int) * 3 * nd, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail("mdk_decoy_coords: upload failed");
+    if (!rc && launch_decoy_coords(d_xyz, na, d_pre, d_rot, d_tr, nd, spacing, offset, d_out, nullptr) != hipSuccess)
+        rc = fail("mdk_decoy_coords: kernel launch failed");
+    if (!rc && hipMemcpy(out, d_out, sizeof(float) * no, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail("mdk_decoy_coords: download failed");
+    (void)hipFree(d_xyz);
+    (void)hipFree(d_pre);
+    (void)hipFree(d_rot);
+    (void)hipFree(d_tr);
+    (void)hipFree(d_out);
+    return rc;
 }
 
 int mdk_copy_probe(int device, size_t bytes, int reps, float *ms) {

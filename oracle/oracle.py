@@ -343,6 +343,14 @@ def ref_rotate(xyz, theta):
     return out
 
 
+def ref_decoygen(new_pdb, ligand_pdb, out_file, n):
+    """The reference's decoygen program (the main of src/decoygen.cpp, compiled into
+    oracle/_ref): writes decoy n of out_file as new_pdb."""
+    argv = [b"decoygen", str(new_pdb).encode(), str(ligand_pdb).encode(), str(out_file).encode(), str(n).encode()]
+    arr = (C.c_char_p * len(argv))(*argv)
+    return ref().main(C.c_int(len(argv)), arr)
+
+
 # ------------------------------------------------------------------ A16 grid sizing
 GRID_TABLE_11 = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 18, 20, 21, 22, 24, 25, 27,
                  28, 30, 32, 33, 35, 36, 40, 42, 44, 45, 48, 49, 50, 54, 55, 56, 60, 63,
