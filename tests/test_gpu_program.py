@@ -53,3 +53,26 @@ def test_program_refuses_a_forced_grid_that_is_too_small(tmp_path, golden_dir):
     l = os.path.join(golden_dir, "1gcq_l.pdb")
     with pytest.raises(M.MdkError, match="too big"):
         M.run(["-R", r, "-L", l, "-o", str(tmp_path / "y.out"), "-F", "24"])
+
+
+def test_entry_points_refuse_invalid_arguments(golden_dir):
+    """Errors come back as a status and a message, with nothing launched: a ligand larger
+    than the context holds, detail translations outside -ng..3ng-1, a non-positive grid
+    width for the device receptor producer, an empty detail call."""
+    lig = M.Protein(os.path.join(golden_dir, "1gcq_l.pdb"))
+    rec = M.Protein(os.path.join(golden_dir, "1gcq_r.pdb"))
+    ng = 48
+    ctx = M.Context(2 * ng, ng, 100, t=1, max_batch=2)
+    with pytest.raises(AssertionError):  # the binding checks the count first
+        ctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, M.grid_coord(ng), 1.2)
+    ctx.set_ligand(lig.xyz[:100], lig.center, lig.radius[:100], lig.charge[:100], M.grid_coord(ng), 1.2)
+    xyz = (rec.xyz - rec.center[None, :]).astype(np.float32)
+    with pytest.raises(M.MdkError, match="invalid grid"):
+        ctx.set_receptor_atoms(xyz, rec.radius, rec.charge, rec.ace, M.grid_coord(ng), gw=0.0)
+    ctx.set_receptor_atoms(xyz, rec.radius, rec.charge, rec.ace, M.grid_coord(ng))
+    rp = np.zeros(ng ** 3, np.float32)
+    rc = M.rotation_coefs(np.zeros((1, 3), np.float32))
+    with pytest.raises(M.MdkError, match="translation out of range"):
+        ctx.score_detail(rp, rc, np.array([[3 * ng, 0, 0]], np.int32))
+    assert ctx.score_detail(rp, rc[:0], np.zeros((0, 3), np.int32)).shape == (0, 6)
+    ctx.close()
