@@ -311,8 +311,17 @@ def main():
                                          "oracle's own mixed-radix FFT (FFTW not installed)"
                                          % (n_cpu, R, R // n_cpu, nf),
                                "seconds": round(cpu_s, 2)}
+        # the global merge over the sample: the n best of its n_cpu * t results, as sets
+        n_top = max(1, n_cpu * t // 4)
+
+        def best(r):
+            r = r.ravel()
+            return {tuple(int(v) for v in r["index"][i])
+                    for i in np.argsort(-r["score"].astype(np.float64), kind="stable")[:n_top]}
+
         out["parity_sample"] = {"rotations": n_cpu, "max_rel_err_top_scores": float(rel.max()),
-                                "best_index_identical": float(np.mean(np.all(g["index"][:, 0] == c["index"][:, 0], axis=1)))}
+                                "best_index_identical": float(np.mean(np.all(g["index"][:, 0] == c["index"][:, 0], axis=1))),
+                                "top_n": n_top, "top_n_overlap": len(best(g) & best(c)) / n_top}
     ctx.close()
     print(json.dumps(out), flush=True)
 

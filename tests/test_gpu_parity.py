@@ -261,6 +261,38 @@ def test_dock_range_matches_batches_and_topn(gpu, gcq, kernels):
     ctx.close()
 
 
+@pytest.mark.parametrize("nf,nrot,t,n", [(96, 360, 1, 100), (144, 240, 3, 200)])
+def test_topn_overlap_with_oracle(gpu, synth, nf, nrot, t, n):
+    """The decoys the program writes: the device's top n over a rotation sample
+    (mdk_dock_range + mdk_topn) against the oracle's loop and merge over the same
+    rotations (src/fft_process.cpp:407-498).  At least 99% of the n (rotation, translation)
+    entries agree as a set, and an entry both keep has the same score within 1e-4."""
+    rec, lig = synth
+    ng = nf // 2
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rr, ri = O.receptor_fft(ng, nf, gr, gi)
+    ang = O.load_rotations(54000)[::54000 // nrot][:nrot]
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=64)
+    ctx.set_receptor(gr, gi)
+    ctx.set_rotations(M.rotation_coefs(ang))
+    _say("top-%d overlap nf=%d: dock %d rotations" % (n, nf, nrot))
+    ctx.dock_range(0, nrot)
+    ctx.sync()
+    top = ctx.topn(nrot, n)
+    ctx.close()
+    _say("top-%d overlap nf=%d: oracle" % (n, nf))
+    ref = O.sort_topn(O.dock(ng, nf, 1.2, lig, rr, ri, ang, np.arange(nrot, dtype=np.int32), t, threads=16), n)
+    got = {tuple(int(v) for v in e["index"]): float(e["score"]) for e in top}
+    exp = {tuple(int(v) for v in e["index"]): float(e["score"]) for e in ref}
+    common = set(got) & set(exp)
+    _say("top-%d overlap nf=%d: %d common" % (n, nf, len(common)))
+    assert len(common) >= 0.99 * n
+    for k in common:
+        assert abs(got[k] - exp[k]) <= 1e-4 * max(abs(exp[k]), 1e-3), (k, got[k], exp[k])
+    # the scores ranked r-th agree as well, entry sets aside
+    assert np.allclose(top["score"], ref["score"], rtol=1e-4, atol=0)
+
+
 @pytest.mark.parametrize("t", [1, 3])
 @pytest.mark.parametrize("ng", [40, 72, 128])
 def test_dock_all_tied_scores(gpu, gcq, kernels, t, ng):
