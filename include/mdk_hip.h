@@ -157,7 +157,8 @@ int mdk_autotune(mdk_ctx *ctx, int first, int count, int *choice);
  * (s, ds/dtx, ds/dty, ds/dtz) where s is the band-limited interpolation of the hard-voxel
  * score map (value / nf^3, equal to the reported score at grid points).  Needs an FFT
  * size with specialised kernels: 64-192 from the reference's size table (see
- * megadock_amd/csrc/mdk_pipe.hip) and 256. */
+ * megadock_amd/csrc/mdk_pipe.hip) and 256.  Any B >= 0; runs in batches of the
+ * context's max_batch. */
 int mdk_score_spectral(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
 
 /* Companion score of B poses (same pose arguments) with the ligand atoms spread
@@ -165,7 +166,8 @@ int mdk_score_spectral(mdk_ctx *ctx, const float *rotcoef, const float *t, int B
  * receptor grids of mdk_set_receptor interpolated at the placed atom P_a = R(a - c) -
  * gw t' (t' the signed translation), w_a = (4/3) pi r_a^3 / gw^3.  out[7b..7b+6] =
  * (s, ds/dt' (3, per voxel), ds/dw (3, per radian of a rotation exp([w]x) applied after
- * R)).  No reference counterpart: the differentiable companion used to refine poses. */
+ * R)).  No reference counterpart: the differentiable companion used to refine poses.
+ * Any B >= 0; runs in batches of the context's max_batch. */
 int mdk_score_companion(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
 
 /* Real-space sums of the reference's detailed output (Docking::output_detail,

@@ -1134,17 +1134,22 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
 int mdk_score_spectral(mdk_ctx *c, const float *rotcoef, const float *t, int B, float *out) {
     if (!c || !rotcoef || !t || !out) return fail("mdk_score_spectral: null argument");
     if (!c->have_rec || !c->have_lig) return fail("mdk_score_spectral: receptor and ligand must be set first");
-    if (B < 1 || B > c->max_batch) return fail("mdk_score_spectral: batch size out of range");
+    if (B < 0) return fail("mdk_score_spectral: negative batch size");
     if (!c->use_pipe)
         return fail("mdk_score_spectral: needs an FFT size with specialised kernels (and MDK_FORCE_GENERIC unset)");
     HIPCHK(hipSetDevice(c->device));
     BatchSlot &S = c->slot[0];
-    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_bt, t, sizeof(float) * 3 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_voxelize(c->lig, c->d_brot, B, S.vs, S.d_vidx, S.d_chg, c->stream));
-    if (run_fft_pipeline(c, S, B, nullptr, nullptr, true)) return 1;
-    HIPCHK(launch_spec_eval(S.d_cc, c->d_bt, B, c->nf, c->H, c->Hp, c->d_gpart, c->d_gout, c->stream));
-    HIPCHK(hipMemcpyAsync(out, c->d_gout, sizeof(float) * 4 * B, hipMemcpyDeviceToHost, c->stream));
+    for (int q0 = 0; q0 < B; q0 += c->max_batch) {  // in batches of the context's size, in order
+        const int nb = std::min(c->max_batch, B - q0);
+        HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef + 9 * (size_t)q0, sizeof(float) * 9 * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_bt, t + 3 * (size_t)q0, sizeof(float) * 3 * nb, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_voxelize(c->lig, c->d_brot, nb, S.vs, S.d_vidx, S.d_chg, c->stream));
+        if (run_fft_pipeline(c, S, nb, nullptr, nullptr, true)) return 1;
+        HIPCHK(launch_spec_eval(S.d_cc, c->d_bt, nb, c->nf, c->H, c->Hp, c->d_gpart, c->d_gout, c->stream));
+        HIPCHK(hipMemcpyAsync(out + 4 * (size_t)q0, c->d_gout, sizeof(float) * 4 * nb, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1153,13 +1158,18 @@ int mdk_score_companion(mdk_ctx *c, const float *rotcoef, const float *t, int B,
     if (!c || !rotcoef || !t || !out) return fail("mdk_score_companion: null argument");
     if (!c->have_recgrid || !c->have_lig)
         return fail("mdk_score_companion: receptor grids (mdk_set_receptor) and ligand must be set first");
-    if (B < 1 || B > c->max_batch) return fail("mdk_score_companion: batch size out of range");
+    if (B < 0) return fail("mdk_score_companion: negative batch size");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef, sizeof(float) * 9 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_bt, t, sizeof(float) * 3 * B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_companion(c->lig, c->d_wshape, c->d_recgrid, c->d_brot, c->d_bt, B, c->nf, c->d_gpart, c->d_gout,
-                            c->stream));
-    HIPCHK(hipMemcpyAsync(out, c->d_gout, sizeof(float) * 7 * B, hipMemcpyDeviceToHost, c->stream));
+    for (int q0 = 0; q0 < B; q0 += c->max_batch) {  // in batches of the context's size, in order
+        const int nb = std::min(c->max_batch, B - q0);
+        HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef + 9 * (size_t)q0, sizeof(float) * 9 * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_bt, t + 3 * (size_t)q0, sizeof(float) * 3 * nb, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_companion(c->lig, c->d_wshape, c->d_recgrid, c->d_brot, c->d_bt, nb, c->nf, c->d_gpart,
+                                c->d_gout, c->stream));
+        HIPCHK(hipMemcpyAsync(out + 7 * (size_t)q0, c->d_gout, sizeof(float) * 7 * nb, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -115,23 +115,21 @@ def refine_hard(dock, rotcoef, t0, steps=20, step_t=0.25, step_w=0.02, min_step=
         gt, gw = tv.grad, wv.grad
         nt = torch.linalg.norm(gt, dim=1).clamp_min(1e-30)
         nw = torch.linalg.norm(gw, dim=1).clamp_min(1e-30)
-        cur = s.detach()
-        pending = torch.ones(B, dtype=torch.bool)
-        while bool(pending.any()):
-            t_new = torch.where(pending[:, None], t + (st / nt)[:, None] * gt, t)
-            R_new = torch.where(pending[:, None, None], rotvec_matrix((sw / nw)[:, None] * gw) @ R, R)
+        cur = s.detach().clone()
+        idx = torch.arange(B)  # decoys still trying this step; only they are evaluated
+        while idx.numel() > 0:
+            t_try = t[idx] + (st[idx] / nt[idx])[:, None] * gt[idx]
+            R_try = rotvec_matrix((sw[idx] / nw[idx])[:, None] * gw[idx]) @ R[idx]
             with torch.no_grad():
-                s_new = SpectralScore.apply(t_new, R_new, dock)
-            ok = pending & (s_new > cur)
-            t = torch.where(ok[:, None], t_new, t)
-            R = torch.where(ok[:, None, None], R_new, R)
-            cur = torch.where(ok, s_new, cur)
-            st = torch.where(ok, st * 1.2, st)
-            sw = torch.where(ok, sw * 1.2, sw)
-            fail = pending & ~ok
-            st = torch.where(fail, st * 0.5, st)
-            sw = torch.where(fail, sw * 0.5, sw)
-            pending = fail & (st >= min_step)
+                s_try = SpectralScore.apply(t_try, R_try, dock)
+            ok = s_try > cur[idx]
+            acc, rej = idx[ok], idx[~ok]
+            t[acc], R[acc], cur[acc] = t_try[ok], R_try[ok], s_try[ok]
+            st[acc] *= 1.2
+            sw[acc] *= 1.2
+            st[rej] *= 0.5
+            sw[rej] *= 0.5
+            idx = rej[st[rej] >= min_step]
     return R.reshape(-1, 9).numpy().astype(np.float32), t.numpy().astype(np.float32), np.array(history)
 
 
@@ -160,22 +158,20 @@ def refine(dock, rotcoef, t0, steps=30, step_t=0.25, step_w=0.02, min_step=1e-4)
         gt, gw = tv.grad, wv.grad
         nt = torch.linalg.norm(gt, dim=1).clamp_min(1e-30)
         nw = torch.linalg.norm(gw, dim=1).clamp_min(1e-30)
-        cur = s.detach()
-        pending = torch.ones(B, dtype=torch.bool)
-        while bool(pending.any()):  # backtracking, per decoy
-            t_new = torch.where(pending[:, None], t + (st / nt)[:, None] * gt, t)
-            w_new = torch.where(pending[:, None], w + (sw / nw)[:, None] * gw, w)
+        cur = s.detach().clone()
+        idx = torch.arange(B)  # backtracking, per decoy: only the decoys still trying are evaluated
+        while idx.numel() > 0:
+            t_try = t[idx] + (st[idx] / nt[idx])[:, None] * gt[idx]
+            w_try = w[idx] + (sw[idx] / nw[idx])[:, None] * gw[idx]
             with torch.no_grad():
-                s_new = CompanionScore.apply(w_new, t_new, base, dock)
-            ok = pending & (s_new > cur)
-            t = torch.where(ok[:, None], t_new, t)
-            w = torch.where(ok[:, None], w_new, w)
-            cur = torch.where(ok, s_new, cur)
-            st = torch.where(ok, st * 1.2, st)
-            sw = torch.where(ok, sw * 1.2, sw)
-            fail = pending & ~ok
-            st = torch.where(fail, st * 0.5, st)
-            sw = torch.where(fail, sw * 0.5, sw)
-            pending = fail & (st >= min_step)
+                s_try = CompanionScore.apply(w_try, t_try, base[idx], dock)
+            ok = s_try > cur[idx]
+            acc, rej = idx[ok], idx[~ok]
+            t[acc], w[acc], cur[acc] = t_try[ok], w_try[ok], s_try[ok]
+            st[acc] *= 1.2
+            sw[acc] *= 1.2
+            st[rej] *= 0.5
+            sw[rej] *= 0.5
+            idx = rej[st[rej] >= min_step]
     R = (rotvec_matrix(w) @ base).reshape(-1, 9).numpy().astype(np.float32)
     return R, t.numpy().astype(np.float32), np.array(history)

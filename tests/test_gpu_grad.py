@@ -132,3 +132,24 @@ def test_companion_score_and_gradient_vs_fp64_and_fd(pair):
         assert np.linalg.norm(got[q, 1:4] - fdt) <= 1e-3 * np.linalg.norm(fdt), (got[q, 1:4], fdt)
         assert np.linalg.norm(got[q, 4:7] - fdw) <= 1e-3 * np.linalg.norm(fdw), (got[q, 4:7], fdw)
     ctx.close()
+
+
+def test_scores_over_more_poses_than_the_batch(pair):
+    """Both off-grid scores take any number of poses: 21 poses through a context of batch 8
+    run as batches of 8, 8 and 5 and equal the results of those batches called one by one
+    (same kernels, fixed reduction order: bitwise)."""
+    rec, lig, ng, gr, gi = pair
+    nf = 2 * ng
+    ctx = _ctx(lig, ng, gr, gi, max_batch=8)
+    ang = O.load_rotations(3600)
+    ids = np.arange(5, 3600, 171)[:21]
+    rc = M.rotation_coefs(ang[ids])
+    t = np.random.default_rng(4).uniform(0, nf, (len(ids), 3)).astype(np.float32)
+    _say("off-grid scores of %d poses, batch 8" % len(ids))
+    for fn in (ctx.score_spectral, ctx.score_companion):
+        whole = fn(rc, t)
+        parts = np.concatenate([fn(rc[q:q + 8], t[q:q + 8]) for q in range(0, len(ids), 8)])
+        assert np.array_equal(whole.view(np.uint32), parts.view(np.uint32)), fn.__name__
+        assert np.all(np.isfinite(whole))
+    assert ctx.score_spectral(rc[:0], t[:0]).shape == (0, 4)
+    ctx.close()
