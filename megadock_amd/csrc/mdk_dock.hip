@@ -1107,7 +1107,7 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
         // fastest one can be chosen freely; each is timed twice after one warm-up pass
         float best = 0.f;
         int bx = c->xcorr_variant, bi = c->inv_variant;
-        for (int xv = 3; xv <= 6; xv++)  // 0..2 are far slower on every size measured
+        for (int xv = 3; xv <= 7; xv++)  // 0..2 are far slower on every size measured
             for (int iv = 0; iv <= 3; iv++) {
                 c->xcorr_variant = xv;
                 c->inv_variant = iv;

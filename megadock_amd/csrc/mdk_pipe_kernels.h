@@ -216,7 +216,12 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
 // QA: the folded lines Q share the LDS of S (two more barriers, a third less LDS).
 // SPEC: store the folded spectrum Q itself, cc[b][kx][ky][kz], and skip the inverse x-FFT
 // (the spectral score evaluation reads it).
-template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false, bool SPEC = false>
+// FLAT: the block's L columns are entries c L .. c L + L - 1 of the flat list, ky slowest,
+// of the columns kz >= KZ0 = L floor((nf/2 + 1) / L) (the ones that do not fill a whole
+// chunk) over all ky; the chunked launch covers kz < KZ0, so no padding column is
+// transformed.
+template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false, bool SPEC = false,
+          bool FLAT = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
               const float2 *__restrict__ twg, int H, int Hp, int B) {
@@ -232,14 +237,29 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
     const int c = B_FAST ? blockIdx.y : blockIdx.x, ky = B_FAST ? blockIdx.z : blockIdx.y;
     const int b0 = (B_FAST ? blockIdx.x : blockIdx.z) * RB;
-    const int kyp = (N - ky) % N;
+    // column (cky, ckz) of line l < L, and whether it exists
+    constexpr int KZ0 = FLAT ? (N / 2 + 1) / L * L : 0, TWN = FLAT ? (N / 2 + 1) % L : 1;
+    static_assert(!FLAT || (TWN > 0 && !SPEC), "flat list of the remaining columns");
+    auto col = [&](int l, int &cky, int &ckz) {
+        if constexpr (FLAT) {
+            const int f = c * L + l;
+            cky = f / TWN;
+            ckz = KZ0 + (f - cky * TWN);
+        } else {
+            cky = ky;
+            ckz = c * L + l;
+        }
+    };
     // fold elements of this thread: column l (fixed), rows kx = fx0 + FX p, p < PER
     static_assert(NT % L == 0, "fold mapping");
     constexpr int FX = NT / L;
-    const int fl = tid % L, fx0 = tid / L, fkz = c * L + fl;
-    const bool fcol = fkz < H;
-    const unsigned fa0 = 8u * (unsigned)((fx0 * N + ky) * N + fkz);                    // + 8 FX N^2 p
-    const unsigned fd0 = 8u * (unsigned)(((N - fx0) * N + kyp) * N + (N - fkz) % N);  // - 8 FX N^2 p
+    const int fl = tid % L, fx0 = tid / L;
+    int fky, fkz;
+    col(fl, fky, fkz);
+    const bool fcol = FLAT ? fky < N : fkz < H;
+    const int fkyp = (N - fky) % N;
+    const unsigned fa0 = 8u * (unsigned)((fx0 * N + fky) * N + fkz);                    // + 8 FX N^2 p
+    const unsigned fd0 = 8u * (unsigned)(((N - fx0) * N + fkyp) * N + (N - fkz) % N);  // - 8 FX N^2 p
     // receptor values of this thread's fold elements (kept in registers when CACHE_R)
     auto rec_pair = [&](int p, float2 &a, float2 &d) {
         a = d = make_float2(0.f, 0.f);
@@ -256,9 +276,10 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
         for (int p = 0; p < PER; p++) rec_pair(p, r1[p], r2[p]);
     }
     const int line = tid % L2, u = tid / L2;
-    const int kzl = c * L + (line % L);
-    const bool valid_line = kzl < H;
-    const int kyl = line < L ? ky : kyp;
+    int lky, kzl;
+    col(line % L, lky, kzl);
+    const bool valid_line = FLAT ? lky < N : kzl < H;
+    const int kyl = line < L ? lky : (N - lky) % N;
     const int kzs = line < L ? kzl : (N - kzl) % N;
 #pragma unroll 1
     for (int bb = 0; bb < RB; bb++) {
@@ -311,7 +332,7 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
             if (fkz < Hp)
 #pragma unroll
                 for (int p = 0; p < PER; p++)
-                    if (fx0 + FX * p < N) st_at(dst, 8u * (unsigned)(((fx0 + FX * p) * N + ky) * Hp + fkz), qv[p]);
+                    if (fx0 + FX * p < N) st_at(dst, 8u * (unsigned)(((fx0 + FX * p) * N + fky) * Hp + fkz), qv[p]);
             continue;
         }
         if (QA) __syncthreads();  // every read of S above is done before Q overwrites it
@@ -336,10 +357,11 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
         if (act && uq < N1) {
             float2 z[N2];
             rf::step2_load<N, true>(z, S, LS, lq, uq);
-            const int kz = c * L + lq;
-            if (kz < Hp) {
+            int sky, kz;
+            col(lq, sky, kz);
+            if (FLAT ? sky < N : kz < Hp) {
                 float2 *dst = cc + (size_t)b * N * N * Hp;  // uniform base, 32-bit lane offsets
-                const unsigned o0 = 8u * (unsigned)((uq * N + ky) * Hp + kz);
+                const unsigned o0 = 8u * (unsigned)((uq * N + sky) * Hp + kz);
                 sfor<0, N2>([&](auto kc) {
                     constexpr int k2 = decltype(kc)::value;
                     st_at(dst, o0 + 8u * (unsigned)(N1 * k2 * N * Hp), z[k2]);
@@ -770,7 +792,8 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     // loaded first (6, the default) or where the fold needs them (5), 8 columns (4); 8
     // columns with a separate fold buffer
     // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
-    // of 4 rotations (1); one rotation per workgroup, tile index fastest (2)
+    // of 4 rotations (1); one rotation per workgroup, tile index fastest (2); as 6 without
+    // the padding columns (7)
     if constexpr (!FULL) {
         kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
             a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
@@ -797,6 +820,16 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
             kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
                 a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
+        case 7: {  // as 6 over the whole chunks of 16 columns, the remaining columns as a flat list
+            constexpr int LC = 2 * LX, NCH = (N / 2 + 1) / LC, NFL = ((N / 2 + 1) % LC * N + LC - 1) / LC;
+            if constexpr (NCH > 0)
+                kp_x_corr<N, LC, 1, true, true, true><<<dim3(B, NCH, N), 2 * LC * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
+                                                                                           a.H, a.Hp, B);
+            if constexpr (NFL > 0)
+                kp_x_corr<N, LC, 1, true, true, true, false, true><<<dim3(B, NFL, 1), 2 * LC * T, 0, s>>>(
+                    a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+            break;
+        }
         default:  // 6: as 5 with the receptor tile loaded into registers before the forward x-FFT,
                   // so that its latency hides behind the transform
             kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(

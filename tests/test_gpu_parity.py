@@ -354,13 +354,14 @@ def test_topn_device_merge_all_tied(gpu, gcq):
     ctx.close()
 
 
-@pytest.mark.parametrize("ng,env", [(48, {"MDK_XCORR_VARIANT": v}) for v in "0123456"] +
+@pytest.mark.parametrize("ng,env", [(48, {"MDK_XCORR_VARIANT": v}) for v in "01234567"] +
                          [(48, {"MDK_INV_VARIANT": v}) for v in "123"] +
                          [(48, {"MDK_STREAMS": "1"}), (48, {"MDK_FWD_VARIANT": "1"})] +
-                         [(72, {"MDK_XCORR_VARIANT": v}) for v in "456"] + [(72, {"MDK_FWD_VARIANT": "1"})])
+                         [(72, {"MDK_XCORR_VARIANT": v}) for v in "4567"] + [(72, {"MDK_FWD_VARIANT": "1"})] +
+                         [(40, {"MDK_XCORR_VARIANT": "7"}), (128, {"MDK_XCORR_VARIANT": "7"})])
 def test_tuning_variants_give_identical_results(gpu, gcq, ng, env):
     """Every kernel variant computes the same operations in the same order: results are
-    bitwise equal to the default's (nf = 96 and 144)."""
+    bitwise equal to the default's (nf = 80, 96, 144 and 256)."""
     rec, lig, _ = gcq
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     ang = O.load_rotations(3600)[:300]
