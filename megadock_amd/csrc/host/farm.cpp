@@ -12,9 +12,10 @@
 //   prep threads (-pj, default min(8, cores/2)) do each job's CPU work: command line,
 //     PDB reading, grid sizing, the receptor's shape/electrostatics grids (cached per
 //     receptor file and grid, since a table repeats receptors), rotation coefficients;
-//   the calling thread docks the prepared jobs in turn on GPU 0 through libmdk contexts
-//     kept per FFT size (re-created only when a job needs more ligand atoms, another t or
-//     another batch) and writes each job's .out file.
+//   the calling thread docks the prepared jobs on GPU 0 through libmdk contexts kept per
+//     FFT size (re-created only when a job needs more ligand atoms, another t or another
+//     batch; at most six alive, the least recently used freed first), taking first a
+//     prepared job whose size has a live context, and writes each job's .out file.
 // A failed job is put back in the queue until it has run 1 + retries times
 // (src/mpidp.cpp:372-543); the log ends with the per-job table (src/mpidp.cpp:900-935).
 #include <sys/time.h>
@@ -286,10 +287,15 @@ struct Slot {
     mdk_ctx *ctx = nullptr;
     int na = 0, t = 0, batch = 0;
     int rot_set = -1;  // rotation set loaded into the context
+    long last_use = 0;
 };
 
+// One context per FFT size, at most kMaxLive at a time (each holds up to two batches of
+// about 8 GiB of buffers): a table with more sizes than that frees the least recently
+// used context before it creates another.
 class Contexts {
 public:
+    static constexpr int kMaxLive = 6;
     ~Contexts() {
         for (auto &kv : slots_)
             if (kv.second.ctx) mdk_destroy(kv.second.ctx);
@@ -300,12 +306,16 @@ public:
         int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
         batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
         Slot &s = slots_[nf];
+        s.last_use = ++clock_;
         if (s.ctx && (s.na < na || s.t != t || s.batch < batch)) {
             mdk_destroy(s.ctx);
             s = Slot();
         }
         if (!s.ctx) {
-            if (mdk_create(0, nf, j.ng, na, t, batch, &s.ctx)) {
+            free_others(kMaxLive - 1, nf);
+            int rc = mdk_create(0, nf, j.ng, na, t, batch, &s.ctx);
+            if (rc && free_others(0, nf) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s.ctx);  // out of memory?
+            if (rc) {
                 err = std::string("mdk_create: ") + mdk_last_error();
                 s = Slot();
                 return nullptr;
@@ -313,13 +323,42 @@ public:
             s.na = na;
             s.t = t;
             s.batch = batch;
+            s.last_use = clock_;
         }
         return s.ctx;
     }
     int &rot_set(int nf) { return slots_[nf].rot_set; }
+    bool has(int nf) const {
+        auto it = slots_.find(nf);
+        return it != slots_.end() && it->second.ctx != nullptr;
+    }
+    int live() const {
+        int n = 0;
+        for (const auto &kv : slots_) n += kv.second.ctx != nullptr;
+        return n;
+    }
 
 private:
+    // destroy the least recently used contexts other than size `keep_nf` until at most
+    // `keep` of them are left; returns how many were destroyed
+    int free_others(int keep, int keep_nf) {
+        int freed = 0;
+        for (;;) {
+            int n = 0;
+            Slot *lru = nullptr;
+            for (auto &kv : slots_)
+                if (kv.first != keep_nf && kv.second.ctx) {
+                    n++;
+                    if (!lru || kv.second.last_use < lru->last_use) lru = &kv.second;
+                }
+            if (n <= keep || !lru) return freed;
+            mdk_destroy(lru->ctx);
+            *lru = Slot();
+            freed++;
+        }
+    }
     std::map<int, Slot> slots_;
+    long clock_ = 0;
 };
 
 int dock(Job &j, Contexts &cs) {
@@ -368,6 +407,17 @@ public:
         if (q_.empty()) return false;
         v = std::move(q_.front());
         q_.pop_front();
+        return true;
+    }
+    // as pop, but takes the first item for which pred holds when there is one
+    template <class P> bool pop_prefer(T &v, P pred) {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return !q_.empty() || closed_; });
+        if (q_.empty()) return false;
+        auto it = std::find_if(q_.begin(), q_.end(), pred);
+        if (it == q_.end()) it = q_.begin();
+        v = std::move(*it);
+        q_.erase(it);
         return true;
     }
     void close() {
@@ -494,7 +544,9 @@ int run_table(int argc, char **argv, std::string &err) {
     {
         Contexts cs;
         std::unique_ptr<Job> j;
-        while (ready.pop(j)) {
+        // a prepared job whose FFT size has a live context goes first: fewer contexts made
+        // and freed when the table mixes more sizes than stay alive
+        while (ready.pop_prefer(j, [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); })) {
             bool ok = false;
             try {
                 ok = dock(*j, cs) == 0;

@@ -520,10 +520,11 @@ template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t
 }
 
 // the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS and gives
-// every thread one task of the y transform
+// every thread at most two tasks of each step of the y transform
 constexpr int plane_stride(int n) { return ((n / 2 + 1) % 2) ? n / 2 + 1 : n / 2 + 2; }
 template <int N> constexpr bool fused_inverse() {
-    return (size_t)N * plane_stride(N) * 8 <= 150 * 1024 && (N / 2 + 1) * Geo<N>::N1 <= 1024;
+    return (size_t)N * plane_stride(N) * 8 <= 150 * 1024 && (N / 2 + 1) * Geo<N>::N1 <= 2 * 1024 &&
+           (N / 2 + 1) * Geo<N>::N2 <= 2 * 1024;
 }
 // the unfused inverse walks a plane in groups of 16 rows
 template <int N> constexpr bool pipe_size_ok() { return N % 2 == 0 && (fused_inverse<N>() || N % 16 == 0); }
@@ -553,8 +554,10 @@ __global__ void __launch_bounds__(1024)
     constexpr int LSP = plane_stride(N);  // row stride of the plane (odd)
     constexpr int N1y = Geo<N>::N1, N2y = Geo<N>::N2;
     constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2;
-    constexpr int KZ = (N * N1z + NTH - 1) / NTH;  // z tasks per thread
-    static_assert(H * N2y <= NTH && H * N1y <= NTH, "one y task per thread");
+    constexpr int KZ = (N * N1z + NTH - 1) / NTH;    // z step-2 tasks per thread
+    constexpr int KZ1 = (N * N2z + NTH - 1) / NTH;   // z step-1 tasks per thread (u < N2z)
+    constexpr int KY1 = (H * N2y + NTH - 1) / NTH, KY2 = (H * N1y + NTH - 1) / NTH;  // y tasks per thread
+    static_assert(!PF || (KY1 == 1 && KY2 == 1), "the prefetch holds one y task");
     __shared__ float2 P[N * LSP];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
@@ -603,17 +606,40 @@ __global__ void __launch_bounds__(1024)
             sfor<0, N1y>([&](auto jc) { v[decltype(jc)::value] = vn[decltype(jc)::value]; });
             rf::step1_g<N, true>(v, P, tw, 1, LSP, 1, yl, yu);
         }
+        // sizes with more step-1 tasks than threads: the further tasks tid + j NTH
+#pragma unroll
+        for (int j = 1; j < KY1; j++) {
+            const int task = tid + j * NTH, ylj = task % H, yuj = task / H;
+            if (yuj < N2y) {
+                const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
+                const unsigned off = (unsigned)(yuj * Hp + ylj);
+                float2 v[N1y];
+                sfor<0, N1y>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    v[j1] = src[off + (unsigned)(N2y * j1 * Hp)];
+                });
+                rf::step1_g<N, true>(v, P, tw, 1, LSP, 1, ylj, yuj);
+            }
+        }
         if (PF) load_plane(p + p_step, yl, yu);
         __syncthreads();
         {
-            float2 y[N2y];
-            if (yu < N1y) rf::step2_g<N, true>(y, P, LSP, 1, yl, yu);
+            float2 y[KY2][N2y];
+#pragma unroll
+            for (int j = 0; j < KY2; j++) {
+                const int task = tid + j * NTH, ylj = task % H, yuj = task / H;
+                if (yuj < N1y) rf::step2_g<N, true>(y[j], P, LSP, 1, ylj, yuj);
+            }
             __syncthreads();
-            if (yu < N1y)
-                sfor<0, N2y>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    P[(yu + N1y * k2) * LSP + yl] = y[k2];
-                });
+#pragma unroll
+            for (int j = 0; j < KY2; j++) {
+                const int task = tid + j * NTH, ylj = task % H, yuj = task / H;
+                if (yuj < N1y)
+                    sfor<0, N2y>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        P[(yuj + N1y * k2) * LSP + ylj] = y[j][k2];
+                    });
+            }
         }
         __syncthreads();
         // ---- M-point inverse of each row (line = ky, element stride 1, line stride LSP),
@@ -622,9 +648,9 @@ __global__ void __launch_bounds__(1024)
         // zero-initialised so that no value of the arrays is carried around the plane loop
         float2 y[KZ][N2z] = {};
         {
-            float2 v[KZ][N1z] = {};
+            float2 v[KZ1][N1z] = {};
 #pragma unroll
-            for (int j = 0; j < KZ; j++) {
+            for (int j = 0; j < KZ1; j++) {
                 const int task = tid + j * NTH, line = task % N, u = task / N;
                 if (u < N2z)
                     sfor<0, N1z>([&](auto jc) {
@@ -640,7 +666,7 @@ __global__ void __launch_bounds__(1024)
             }
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < KZ; j++) {
+            for (int j = 0; j < KZ1; j++) {
                 const int task = tid + j * NTH, line = task % N, u = task / N;
                 if (u < N2z) rf::step1_g<M, true>(v[j], P, tw, 2, 1, LSP, line, u);
             }
@@ -722,9 +748,7 @@ template <int N, int TT, bool FULL> void launch_inv_yz(const PipeArgs &a, hipStr
     const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
     if constexpr (!FULL) {
         kp_inv_yz_top<N, TT, false, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
-        return;
-    }
-    switch (a.inv_variant) {
+    } else switch (a.inv_variant) {
         case 1:
             kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
             break;

@@ -55,3 +55,27 @@ def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):
     assert rows["00002"][1:3] == ["3", "failed"]  # the first run and two retries
     assert (tmp_path / "ok.out").exists()
     assert len((tmp_path / "ok.out").read_text().splitlines()) == 4 + 10
+
+
+def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
+    """Eight FFT sizes and then two of the first ones again: the farm keeps at most six
+    contexts alive, frees the least recently used one for each new size, and makes a
+    context again (with its rotation table) when an evicted size comes back.  Every job's
+    .out equals the single-pair program's."""
+    r = os.path.join(golden_dir, "1gcq_r.pdb")
+    l = os.path.join(golden_dir, "1gcq_l.pdb")
+    sizes = [80, 84, 88, 90, 96, 98, 100, 108, 80, 88]
+    lines = ["%s\t%s\t%s\t%d" % (r, l, tmp_path / ("farm_%d.out" % k), nf) for k, nf in enumerate(sizes)]
+    tb = tmp_path / "jobs.table"
+    tb.write_text("TITLE=sizes\nPARAM=-R $1 -L $2 -o $3 -F $4 -r 24 -N 20 -W 8\n" + "\n".join(lines) + "\n")
+    log = tmp_path / "master.log"
+    print("farm of %d jobs over %d sizes" % (len(sizes), len(set(sizes))), flush=True)
+    M.run_table(["-tb", str(tb), "-lg", str(log), "-pj", "2"])
+    rows = _rows(log)
+    assert all(rows["%05d" % (k + 1)][2] == "done" for k in range(len(sizes)))
+    for k, nf in enumerate(sizes):
+        single = tmp_path / ("single_%d.out" % k)
+        M.run(["-R", r, "-L", l, "-o", str(single), "-F", str(nf), "-r", "24", "-N", "20", "-W", "8"])
+        got = (tmp_path / ("farm_%d.out" % k)).read_bytes()
+        assert got == single.read_bytes(), (k, nf)
+        assert got.split(b"\t")[0] == str(nf).encode()
