@@ -140,15 +140,17 @@ def bench_c3(args, M, workloads):
     """c3: the best 2000 decoys of c1 refined off the grid on the hard-voxel score
     (megadock_amd.refine.refine_hard: translation along the exact gradient of the
     band-limited hard score, rotation along the trilinear companion's torque, a step kept
-    only when the hard score rises).  value = pose steps per second of the refinement."""
+    only when the hard score rises).  value = pose steps per second of the refinement.
+    --refine-of c4 --refine-top 10000 gives c4's refinement of its best 10000 decoys."""
     from megadock_amd import refine
 
-    rec_pdb, lig_pdb, cfg = workloads.make_pair("c1", args.cache)
+    rec_pdb, lig_pdb, cfg = workloads.make_pair(args.refine_of, args.cache)
     rec, lig = M.Protein(rec_pdb), M.Protein(lig_pdb)
     nf = cfg["nf"]
     ng, R, t = nf // 2, cfg["rotations"], cfg["t"]
     gr, gi = M.receptor_grids(rec, ng)
-    ctx = M.Context(nf, ng, lig.na, t=t, max_batch=128)
+    per = 8.0 * (ng * ng * nf + ng * nf * nf + nf * nf * (nf // 2 + 17) + nf * nf * t)
+    ctx = M.Context(nf, ng, lig.na, t=t, max_batch=max(1, min(128, int(8 * 2 ** 30 / per))))
     ctx.set_receptor(gr, gi)
     ctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, M.grid_coord(ng), 1.2)
     angles = np.loadtxt(os.path.join(M.DATA, "rotations_%d.txt" % R), dtype=np.float64).astype(np.float32)
@@ -158,7 +160,7 @@ def bench_c3(args, M, workloads):
     ctx.dock_range(0, R)
     ctx.sync()
     dock_s = time.perf_counter() - td
-    top = ctx.topn(R, cfg["n_out"])
+    top = ctx.topn(R, args.refine_top)
     n = len(top)
     rc = rot[top["index"][:, 0]]
     t0 = top["index"][:, 1:].astype(np.float32)
@@ -189,11 +191,12 @@ def bench_c3(args, M, workloads):
     scale = np.maximum(np.abs(top["score"].astype(np.float64)), 1e-3)
     gain = (h1 - h0) / np.maximum(np.abs(h0), 1e-3)
     print(json.dumps({
-        "metric": "c3: off-grid refinement of the 2000 best c1 decoys, pose steps/s",
+        "metric": "c3: off-grid refinement of the best decoys, pose steps/s",
         "value": round(n * args.refine_steps / ref_s, 1), "unit": "pose steps/s", "n_gpus": 1,
         "higher_is_better": True, "dtype": "f32 (scores), f64 (pose algebra on the host)", "data": "synthetic",
-        "config": {"workload": "c3: best %d decoys of c1 (144^3, 54000 rotations, t = 3), %d refinement steps"
-                               % (n, args.refine_steps), "decoys": n, "steps": args.refine_steps},
+        "config": {"workload": "c3: best %d decoys of %s (%d^3, %d rotations, t = %d), %d refinement steps"
+                               % (n, args.refine_of, nf, R, t, args.refine_steps), "decoys": n,
+                   "steps": args.refine_steps},
         "refine_seconds": round(ref_s, 3), "dock_seconds": round(dock_s, 3),
         "pose_evaluations": {"spectral": dock.spectral, "companion": dock.companion,
                              "per_second": round((dock.spectral + dock.companion) / ref_s, 1)},
@@ -214,6 +217,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--refine-steps", type=int, default=10, help="c3: refinement steps")
+    ap.add_argument("--refine-of", default="c1", choices=["c1", "c4"], help="c3: the docking run whose decoys are refined")
+    ap.add_argument("--refine-top", type=int, default=2000, help="c3: number of decoys refined")
     ap.add_argument("--no-sequential", action="store_true", help="c2: skip the one-after-another comparison")
     ap.add_argument("--pairs", default="4x8", help="c2: receptors x ligands of the 32 x 32 table to run (first ones)")
     ap.add_argument("--batch", type=int, default=0, help="rotations per device batch (0: automatic)")

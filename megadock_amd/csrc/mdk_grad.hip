@@ -44,25 +44,26 @@ __global__ void __launch_bounds__(256) k_spec_eval(const float2 *__restrict__ Q,
     }
     __syncthreads();
     const float2 *q = Q + (size_t)b * nf * nf * Hp;
-    // lane owns kz = lane and lane + 64 (< H)
-    float2 ez[2];
-    float wz[2], kzs[2];
-    bool on[2];
-    for (int j = 0; j < 2; j++) {
+    // lane owns kz = lane, lane + 64 and lane + 128 (< H; kz = 128 is the Nyquist plane of nf = 256)
+    constexpr int JZ = 3;
+    float2 ez[JZ];
+    float wz[JZ], kzs[JZ];
+    bool on[JZ];
+    for (int j = 0; j < JZ; j++) {
         const int kz = lane + 64 * j;
         on[j] = kz < H;
         ez[j] = on[j] ? ph[2 * nf + kz] : make_float2(0.f, 0.f);
         wz[j] = (kz == 0 || 2 * kz == nf) ? 1.0f : 2.0f;
         kzs[j] = (float)signed_freq(kz, nf);
     }
-    float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;  // 8.5M terms a pose at 256^3: summed in double
     const int rows = nf * nf, wtot = kSpecBlocks * 4;
     for (int r = blockIdx.x * 4 + wv; r < rows; r += wtot) {
         const int kx = r / nf, ky = r - kx * nf;
         const float2 ex = ph[kx], ey = ph[nf + ky];
         const float2 er = make_float2(ex.x * ey.x - ex.y * ey.y, ex.x * ey.y + ex.y * ey.x);
         const float kxs = (float)signed_freq(kx, nf), kys = (float)signed_freq(ky, nf);
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < JZ; j++) {
             if (!on[j]) continue;
             const float2 e = make_float2(er.x * ez[j].x - er.y * ez[j].y, er.x * ez[j].y + er.y * ez[j].x);
             const float2 x = q[(size_t)r * Hp + lane + 64 * j];

@@ -62,16 +62,23 @@ def test_spectral_score_and_gradient_vs_fp64(pair, ngs):
     ctx.close()
 
 
-def test_spectral_score_at_grid_points_is_the_docked_score(pair):
+@pytest.mark.parametrize("ngs", [None, 128])
+def test_spectral_score_at_grid_points_is_the_docked_score(pair, ngs):
+    """nf = 96 and 256 (whose kz = 128 Nyquist plane enters the sum with weight 1): the
+    band-limited score at each docked translation equals the docked score."""
     rec, lig, ng, gr, gi = pair
-    ctx = _ctx(lig, ng, gr, gi, t=1)
+    if ngs is not None:
+        ng = ngs
+        gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    ctx = _ctx(lig, ng, gr, gi, t=3)
     ang = O.load_rotations(3600)
     ids = np.array([5, 777, 3000], np.int32)
     rc = M.rotation_coefs(ang[ids])
+    _say("spectral at docked grid points nf=%d" % (2 * ng))
     res = ctx.dock(rc, ids)
     t = res["index"][:, 1:].astype(np.float32)
-    got = ctx.score_spectral(rc, t)
-    for q in range(len(ids)):
+    got = ctx.score_spectral(np.repeat(rc, 3, axis=0), t)
+    for q in range(len(res)):
         assert abs(got[q, 0] - res["score"][q]) <= 1e-5 * max(abs(res["score"][q]), 1e-3), (q, got[q], res[q])
     ctx.close()
 
