@@ -1137,6 +1137,7 @@ int mdk_score_spectral(mdk_ctx *c, const float *rotcoef, const float *t, int B, 
     if (B < 0) return fail("mdk_score_spectral: negative batch size");
     if (!c->use_pipe)
         return fail("mdk_score_spectral: needs an FFT size with specialised kernels (and MDK_FORCE_GENERIC unset)");
+    if (c->H > 192) return fail("mdk_score_spectral: nf above 382 (three kz per lane in the evaluation)");
     HIPCHK(hipSetDevice(c->device));
     BatchSlot &S = c->slot[0];
     for (int q0 = 0; q0 < B; q0 += c->max_batch) {  // in batches of the context's size, in order
