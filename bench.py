@@ -261,7 +261,8 @@ def main():
     if B <= 0:
         H = nf // 2 + 1
         per = 8.0 * (ng * ng * nf + ng * nf * nf + nf * nf * (H + 16) + nf * nf * t)
-        B = max(1, min(256 if nf <= 128 else 128, int(8 * 2 ** 30 / per)))  # as the host layer's auto_batch
+        cap = 256 if nf <= 128 else 128 if nf <= 192 else 192  # as the host layer's auto_batch
+        B = max(1, min(cap, int((36 if nf > 192 else 8) * 2 ** 30 / per)))
     ctx = M.Context(nf, ng, lig.na, t=t, max_batch=B)
     ctx.set_receptor(gr, gi)
     ctx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, M.grid_coord(ng), 1.2)
