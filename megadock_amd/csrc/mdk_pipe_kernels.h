@@ -429,21 +429,26 @@ __device__ __forceinline__ void block_top(TopR<TT> &tl, int t, Cand *wl, Cand *d
 }
 
 // ------------------------------------------------------------------ inverse z + top-t
-// block (x, b): the N rows r = x*N + y of plane x of rotation b, L rows at a time; the
-// row's spectrum is Hermitian in kz, only kz <= N/2 is stored.  Every thread keeps a
-// running top-TT (TT >= t, compile time) of its outputs over the plane's row groups; the
-// lists are merged across each wavefront with lane shuffles and across wavefronts through
-// LDS once per plane, leaving the plane's top-t as t candidates.  The test hook writes the
-// real score rows to smap instead.
+// block (x, b): the N rows r = x*N + y of plane x of rotation b, L rows at a time; a row's
+// spectrum is Hermitian in kz and only kz <= N/2 is stored.  As in the fused inverse, a row
+// goes through the half-length complex-to-real step: with M = N/2 and w = exp(+2 pi i / N),
+//   Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])),  k < M,
+// whose M-point inverse gives z[m] = x[2m] + i x[2m+1], two real scores per output.  Every
+// thread keeps a running top-TT (TT >= t, compile time) of its outputs over the plane's row
+// groups; the lists are merged across each wavefront with lane shuffles and across
+// wavefronts through LDS once per plane, leaving the plane's top-t as t candidates.  The
+// test hook writes the real score rows to smap instead.
 template <int N, int L, int TT>
-__global__ void __launch_bounds__(L *Geo<N>::T)
+__global__ void __launch_bounds__(L *Geo<N / 2>::N1)
     kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                  const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
-    constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T, LS = L + 1, NT = L * T;
+    constexpr int M = N / 2, HC = M + 1;
+    constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2, NT = L * N1z;
+    constexpr int LSR = HC % 2 ? HC : HC + 1;  // row stride in LDS (odd)
     constexpr int NROW = N * N, NW = (NT + 63) / 64, RG = N / L;
     static_assert(NT % 64 == 0, "the wavefront top-t needs whole wavefronts");
-    static_assert(N % L == 0, "whole row groups per plane");
-    __shared__ float2 S[N * LS];
+    static_assert(N % L == 0 && N2z <= N1z, "whole row groups per plane, one task per thread and step");
+    __shared__ float2 S[L * LSR];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
@@ -463,60 +468,66 @@ __global__ void __launch_bounds__(L *Geo<N>::T)
     for (int g = 0; g < RG; g++) {
         const int row0 = (blockIdx.x * RG + g) * L;
         __syncthreads();  // S is free: the previous row group has read it
-        for (int q = tid; q < L * H; q += NT) {
-            const int l = q / H, kz = q - l * H;
-            const float2 v = src[(size_t)(row0 + l) * Hp + kz];
-            S[kz * LS + l] = v;
-            if (kz >= 1 && N - kz > kz) S[(N - kz) * LS + l] = make_float2(v.x, -v.y);
+        for (int q = tid; q < L * HC; q += NT) {
+            const int l = q / HC, kz = q - l * HC;
+            S[l * LSR + kz] = src[(size_t)(row0 + l) * Hp + kz];
         }
         __syncthreads();
-        {
-            float2 v[N1];
-            if (u < N2)
-                sfor<0, N1>([&](auto jc) {
-                    constexpr int j1 = decltype(jc)::value;
-                    v[j1] = S[(u + N2 * j1) * LS + line];
-                });
-            __syncthreads();  // step 1 writes back into S
-            if (u < N2) rf::step1_store<N, true>(v, S, tw, LS, line, u);
-        }
+        float2 v[N1z];
+        if (u < N2z)
+            sfor<0, N1z>([&](auto jc) {
+                constexpr int j1 = decltype(jc)::value;
+                const int k = u + N2z * j1;
+                const float2 a = S[line * LSR + k], c = S[line * LSR + M - k], w = tw[k];
+                const float2 s1 = make_float2(a.x + c.x, a.y - c.y);  // X[k] + conj(X[M-k])
+                const float2 d1 = make_float2(a.x - c.x, a.y + c.y);  // X[k] - conj(X[M-k])
+                // e = conj(w) d1; Z = s1 + i e
+                const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
+                v[j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
+            });
+        __syncthreads();  // every read of S is done before step 1 writes into it
+        if (u < N2z) rf::step1_g<M, true>(v, S, tw, 2, 1, LSR, line, u);
         __syncthreads();
+        float2 y[N2z];
+        rf::step2_g<M, true>(y, S, 1, LSR, line, u);
         const int r = row0 + line;
-        const bool act = u < N1;
-        float2 y[N2];
-        if (u < N1) rf::step2_load<N, true>(y, S, LS, line, u);
+        // z = 2m and 2m+1 of row r are y[k2].x and .y with m = u + N1z k2
         if (smap) {  // uniform
-            if (act)
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    smap[((size_t)b * NROW + r) * N + u + N1 * k2] = y[k2].x;
-                });
+            sfor<0, N2z>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                const size_t o = ((size_t)b * NROW + r) * N + 2 * (u + N1z * k2);
+                smap[o] = y[k2].x;
+                smap[o + 1] = y[k2].y;
+            });
             continue;
         }
         if (g < 4 || (g & 3) == 0) {  // uniform
             float mx = -INFINITY;
-            if (act)
-                sfor<0, N2>([&](auto kc) { mx = fmaxf(mx, y[decltype(kc)::value].x); });
+            sfor<0, N2z>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                mx = fmaxf(mx, fmaxf(y[k2].x, y[k2].y));
+            });
             const float tau = wave_kth_max(mx, t);
             if (tau > run_tau) {  // uniform
                 run_tau = tau;
                 cut = QUOT ? quotient_cut(tau, nf3f) : tau;
             }
         }
-        if (act)
-            sfor<0, N2>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                const float v = y[k2].x;
-                if (v >= cut) tl.push(QUOT ? v / nf3f : v, r * N + u + N1 * k2);
-            });
+        sfor<0, N2z>([&](auto kc) {
+            constexpr int k2 = decltype(kc)::value;
+            const int z0 = 2 * (u + N1z * k2);
+            const float v0 = y[k2].x, v1 = y[k2].y;
+            if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, r * N + z0);
+            if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, r * N + z0 + 1);
+        });
     }
     if (smap) return;  // uniform
     block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
 template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
-    constexpr int T = Geo<N>::T;
-    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), L * T, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, a.t, a.nf3f);
+    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), L * Geo<N / 2>::N1, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, a.t,
+                                                                       a.nf3f);
 }
 
 // the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS and gives
