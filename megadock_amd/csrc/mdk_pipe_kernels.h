@@ -124,7 +124,9 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
     constexpr int LA = NG + 1, LE = C + 1;
     static_assert(fwd_zy_fits<N, C>(), "fused forward pass does not fit");
-    __shared__ float2 A[N * LA];  // z-transformed plane, [kz][y]
+    // one spare row after each group of N2 in the z exchange, where it still fits
+    constexpr int PZ = (size_t)(N + N1) * LA * 8 + (size_t)N * LE * 8 + (size_t)N * 8 <= 150 * 1024 ? 1 : 0;
+    __shared__ float2 A[(N + N1 * PZ) * LA];  // z-transformed plane, [kz][y]
     __shared__ float2 E[N * LE];  // y-exchange of one chunk of columns
     __shared__ float2 tw[N];
     for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
@@ -165,11 +167,11 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
                         v[j1] = make_float2((d & bit) ? delta : ((o & bit) ? 1.0f : 0.0f), q);
                     }
                 }
-                rf::step1_g<N, false>(v, A, tw, 1, LA, 1, y, u);
+                rf::step1_g<N, false, PZ>(v, A, tw, 1, LA, 1, y, u);
             }
             __syncthreads();
             float2 w[N2] = {};
-            if (zrow && u < N1) rf::step2_g<N, false>(w, A, LA, 1, y, u);
+            if (zrow && u < N1) rf::step2_g<N, false, PZ>(w, A, LA, 1, y, u);
             __syncthreads();
             if (zrow && u < N1)
                 sfor<0, N2>([&](auto kc) {

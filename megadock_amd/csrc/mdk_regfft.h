@@ -150,7 +150,9 @@ template <int N> struct Geo {
 
 // General strides: element e of line `line` is S[e * ES + line * LST]; the N-point
 // twiddle w_N^m is tw[m * TWS] (TWS > 1 when tw is a table for a multiple of N).
-template <int N, bool INV>
+// PAD: PAD spare elements after each group of N2 (the exchange then spans N + N1 PAD
+// elements); it moves the groups of the k1 = 0, 1, ... threads onto other LDS banks.
+template <int N, bool INV, int PAD = 0>
 __device__ __forceinline__ void step1_g(float2 *v, float2 *S, const float2 *tw, int TWS, int ES, int LST, int line,
                                         int j2) {
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2;
@@ -163,16 +165,16 @@ __device__ __forceinline__ void step1_g(float2 *v, float2 *S, const float2 *tw, 
             if (INV) w.y = -w.y;
             x = make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
         }
-        S[(k1 * N2 + j2) * ES + line * LST] = x;
+        S[(k1 * (N2 + PAD) + j2) * ES + line * LST] = x;
     });
 }
 
-template <int N, bool INV>
+template <int N, bool INV, int PAD = 0>
 __device__ __forceinline__ void step2_g(float2 *y, const float2 *S, int ES, int LST, int line, int k1) {
     constexpr int N2 = Geo<N>::N2;
     sfor<0, N2>([&](auto jc) {
         constexpr int j2 = decltype(jc)::value;
-        y[j2] = S[(k1 * N2 + j2) * ES + line * LST];
+        y[j2] = S[(k1 * (N2 + PAD) + j2) * ES + line * LST];
     });
     RDft<N2, INV>::run(y);
 }
