@@ -232,6 +232,8 @@ struct Job {
     std::shared_ptr<const Grids> grids;
     std::string err;
     double t_prep = 0, t_dock = 0;
+    // filled by docking: the merged top-N, written to the .out file afterwards
+    std::vector<SortScore> top;
 };
 
 struct JobRecord {
@@ -361,7 +363,12 @@ private:
     long clock_ = 0;
 };
 
-int dock(Job &j, Contexts &cs) {
+// The device part of a job in two calls: issue() uploads the job and queues its batches
+// (mdk_dock_range returns while they run); complete() waits for the merged top-N, and for
+// -O also scores and writes the detailed output, which needs the context.  The .out file
+// is written by write_out(), from host data only, so the GPU thread can write one job's
+// file while the next job's batches run.
+int issue(Job &j, Contexts &cs) {
     const double t0 = now_s();
     Parameter &p = j.prm;
     mdk_ctx *ctx = cs.get(j, j.err);
@@ -381,14 +388,32 @@ int dock(Job &j, Contexts &cs) {
                              j.gc.data(), p.grid_width, p.lig_core))
         return abi("mdk_set_ligand_atoms");
     if (mdk_dock_range(ctx, 0, p.num_rot_angles)) return abi("mdk_dock_range");
-    std::vector<mdk_sortscore> best(std::max(1, p.num_output));
-    if (mdk_topn(ctx, p.num_rot_angles, p.num_output, best.data())) return abi("mdk_topn");
-    std::vector<SortScore> top(best.size());
-    std::memcpy(top.data(), best.data(), sizeof(SortScore) * best.size());
-    if (write_decoys(p, *j.rec, L, j.ng, top.data(), j.err)) return 1;
-    if (p.detail_output_flag && write_detail(p, *j.rec, L, j.ng, top.data(), ctx, j.err)) return 1;
     j.t_dock = now_s() - t0;
     return 0;
+}
+
+int complete(Job &j, Contexts &cs) {
+    const double t0 = now_s();
+    Parameter &p = j.prm;
+    mdk_ctx *ctx = cs.get(j, j.err);  // the context issue() used, still alive
+    if (!ctx) return 1;
+    std::vector<mdk_sortscore> best(std::max(1, p.num_output));
+    if (mdk_topn(ctx, p.num_rot_angles, p.num_output, best.data())) {
+        j.err = std::string("mdk_topn: ") + mdk_last_error();
+        return 1;
+    }
+    j.top.resize(best.size());
+    std::memcpy(j.top.data(), best.data(), sizeof(SortScore) * best.size());
+    if (p.detail_output_flag && write_detail(p, *j.rec, *j.lig, j.ng, j.top.data(), ctx, j.err)) return 1;
+    j.t_dock += now_s() - t0;
+    return 0;
+}
+
+int write_out(Job &j) {
+    const double t0 = now_s();
+    const int rc = write_decoys(j.prm, *j.rec, *j.lig, j.ng, j.top.data(), j.err);
+    j.t_dock += now_s() - t0;
+    return rc;
 }
 
 template <class T> class Queue {
@@ -407,6 +432,16 @@ public:
         if (q_.empty()) return false;
         v = std::move(q_.front());
         q_.pop_front();
+        return true;
+    }
+    // without waiting: false when the queue is empty, else as pop_prefer
+    template <class P> bool try_pop_prefer(T &v, P pred) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (q_.empty()) return false;
+        auto it = std::find_if(q_.begin(), q_.end(), pred);
+        if (it == q_.end()) it = q_.begin();
+        v = std::move(*it);
+        q_.erase(it);
         return true;
     }
     // as pop, but takes the first item for which pred holds when there is one
@@ -543,20 +578,44 @@ int run_table(int argc, char **argv, std::string &err) {
         });
     {
         Contexts cs;
-        std::unique_ptr<Job> j;
-        // a prepared job whose FFT size has a live context goes first: fewer contexts made
-        // and freed when the table mixes more sizes than stay alive
-        while (ready.pop_prefer(j, [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); })) {
+        std::unique_ptr<Job> j, written;  // written: docked, its .out file still to write
+        auto report = [&](std::unique_ptr<Job> x, bool ok) {
+            std::printf("# job %05d (%s): %s\n", x->id, x->line.c_str(), ok ? "done" : x->err.c_str());
+            std::fflush(stdout);
+            finish(std::move(x), ok);
+        };
+        auto write_pending = [&] {
+            if (!written) return;
             bool ok = false;
             try {
-                ok = dock(*j, cs) == 0;
+                ok = write_out(*written) == 0;
+            } catch (const std::exception &e) {
+                written->err = std::string("output: ") + e.what();
+            }
+            report(std::move(written), ok);
+        };
+        // a prepared job whose FFT size has a live context goes first: fewer contexts made
+        // and freed when the table mixes more sizes than stay alive
+        auto live = [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); };
+        for (;;) {
+            if (!ready.try_pop_prefer(j, live)) {
+                write_pending();  // nothing prepared: finish the previous job before waiting
+                if (!ready.pop_prefer(j, live)) break;
+            }
+            bool ok = false;
+            try {
+                ok = issue(*j, cs) == 0;
+                write_pending();  // the previous job's file, while this job's batches run
+                ok = ok && complete(*j, cs) == 0;
             } catch (const std::exception &e) {
                 j->err = std::string("docking: ") + e.what();
+                ok = false;
             }
-            std::printf("# job %05d (%s): %s\n", j->id, j->line.c_str(), ok ? "done" : j->err.c_str());
-            std::fflush(stdout);
-            finish(std::move(j), ok);
+            write_pending();
+            if (ok) written = std::move(j);
+            else report(std::move(j), false);
         }
+        write_pending();
     }
     for (auto &w : workers) w.join();
 
