@@ -15,7 +15,8 @@
 //   the calling thread docks the prepared jobs on GPU 0 through libmdk contexts kept per
 //     FFT size (re-created only when a job needs more ligand atoms, another t or another
 //     batch; at most six alive, the least recently used freed first), taking first a
-//     prepared job whose size has a live context, and writes each job's .out file.
+//     prepared job whose size has a live context, and writes each job's .out file while
+//     the next job's batches run.
 // A failed job is put back in the queue until it has run 1 + retries times
 // (src/mpidp.cpp:372-543); the log ends with the per-job table (src/mpidp.cpp:900-935).
 #include <sys/time.h>
