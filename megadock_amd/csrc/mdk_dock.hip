@@ -468,8 +468,9 @@ struct mdk_ctx {
     int *d_bid = nullptr;    // staging ids
     // batch buffers: one set per stream; mdk_dock_range alternates batches between the
     // streams so that one batch's memory-bound passes overlap another's compute-bound ones
-    BatchSlot slot[2];
-    int nslots = 1;                // batch slots and streams (MDK_STREAMS: 1 or 2)
+    static constexpr int kMaxSlots = 3;
+    BatchSlot slot[kMaxSlots];
+    int nslots = 1;                // batch slots and streams (MDK_STREAMS: 1 to 3)
     bool use_graph = false;        // MDK_GRAPH: full batches of mdk_dock_range replay a captured graph
     mdk_sortscore *d_results = nullptr; // nrot * t
     // off-grid scores and gradients (mdk_grad.hip)
@@ -591,7 +592,7 @@ int ensure_graph(mdk_ctx *c, BatchSlot &S) {
 }
 
 void drop_graphs(mdk_ctx *c) {
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < mdk_ctx::kMaxSlots; k++) {
         BatchSlot &S = c->slot[k];
         if (S.exec || S.graph) {
             if (S.stream) (void)hipStreamSynchronize(S.stream);
@@ -671,10 +672,11 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail("mdk_create: hipSetDevice failed"); }
     {
         // two batches in flight on two streams (one kernel's tail overlaps the next batch's
-        // first kernels; 61.0k against 59.6k rotations/s at 144^3), unless the batches are
-        // small or MDK_STREAMS asks for 1
+        // first kernels; 61.0k against 59.6k rotations/s at 144^3; a third stream gave
+        // 61.5k-62.8k against 63.0k in full runs), unless the batches are small or
+        // MDK_STREAMS asks for 1 or 3
         const char *ns = std::getenv("MDK_STREAMS");
-        c->nslots = ns ? (std::atoi(ns) == 2 ? 2 : 1) : (max_batch >= 32 ? 2 : 1);
+        c->nslots = ns ? std::max(1, std::min(mdk_ctx::kMaxSlots, std::atoi(ns))) : (max_batch >= 32 ? 2 : 1);
         const char *gr = std::getenv("MDK_GRAPH");
         c->use_graph = gr && gr[0] == '1';
     }
@@ -744,7 +746,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
 int mdk_destroy(mdk_ctx *c) {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < mdk_ctx::kMaxSlots; k++)
         if (c->slot[k].stream) (void)hipStreamSynchronize(c->slot[k].stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
                     c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out,
@@ -752,7 +754,7 @@ int mdk_destroy(mdk_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < mdk_ctx::kMaxSlots; k++) {
         BatchSlot &S = c->slot[k];
         void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout, S.d_ctl};
         for (void *p : sp)
