@@ -719,13 +719,16 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
             return bail(fail("mdk_create: twiddle upload failed"));
         const char *g = std::getenv("MDK_FORCE_GENERIC");
         c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
-        // correlation variant 7 (no padding columns) where variant 6's last chunk of 16
-        // columns holds a single one, up to nf = 128: at 96^3, 164.9-165.7k against
-        // 160.3-163.0k rotations/s (only the tuned sizes have variant 7)
+        // default variants, as measured per tuned size (the further sizes build only 6 and 2):
+        // 96^3 takes correlation variant 7 (no padding columns; variant 6's last chunk of 16
+        // columns holds one: 164.9-165.7k against 160.3-163.0k rotations/s) and inverse
+        // variant 3 (167.7-169.1k against 164.2-165.4k); 144^3 keeps 6 and 2 (inverse
+        // variant 3: 58.8-59.1k against 59.3-59.4k)
+        const int dx = nf == 96 ? 7 : 6, di = nf == 96 ? 3 : 2;
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
-        c->xcorr_variant = xv ? std::atoi(xv) : ((nf / 2 + 1) % 16 == 1 && nf <= 128 ? 7 : 6);
+        c->xcorr_variant = xv ? std::atoi(xv) : dx;
         const char *iv = std::getenv("MDK_INV_VARIANT");
-        c->inv_variant = iv ? std::atoi(iv) : 2;
+        c->inv_variant = iv ? std::atoi(iv) : di;
         const char *fv = std::getenv("MDK_FWD_VARIANT");
         c->fwd_variant = fv ? std::atoi(fv) : 0;
         int cus = 0;
