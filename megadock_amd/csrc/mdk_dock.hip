@@ -722,9 +722,10 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         // default variants, as measured per tuned size (the further sizes build only 6 and 2):
         // 96^3 takes correlation variant 7 (no padding columns; variant 6's last chunk of 16
         // columns holds one: 164.9-165.7k against 160.3-163.0k rotations/s) and inverse
-        // variant 3 (167.7-169.1k against 164.2-165.4k); 144^3 keeps 6 and 2 (inverse
-        // variant 3: 58.8-59.1k against 59.3-59.4k)
-        const int dx = nf == 96 ? 7 : 6, di = nf == 96 ? 3 : 2;
+        // variant 3 (167.7-169.1k against 164.2-165.4k); 80^3 takes inverse variant 3 as well
+        // (293.3-293.8k against 280.0-280.5k; correlation 7 is slower there); 144^3 keeps 6
+        // and 2 (inverse variant 3: 58.8-59.1k against 59.3-59.4k)
+        const int dx = nf == 96 ? 7 : 6, di = nf == 96 || nf == 80 ? 3 : 2;
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
         c->xcorr_variant = xv ? std::atoi(xv) : dx;
         const char *iv = std::getenv("MDK_INV_VARIANT");
