@@ -760,7 +760,10 @@ template <int N, int TT, bool FULL> void launch_inv_yz(const PipeArgs &a, hipStr
     const int nplanes = a.B * N;
     const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
     if constexpr (!FULL) {
-        kp_inv_yz_top<N, TT, false, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+        // the further sizes: running lists, with the next plane prefetched (variant 3) up to
+        // nf = 128 where a thread holds one y task, as measured best at 80^3 and 96^3
+        constexpr bool PF = N <= 128 && (N / 2 + 1) * Geo<N>::N1 <= 1024 && (N / 2 + 1) * Geo<N>::N2 <= 1024;
+        kp_inv_yz_top<N, TT, PF, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
     } else switch (a.inv_variant) {
         case 1:
             kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
@@ -790,7 +793,8 @@ template <int N> constexpr int fwd_zy_chunk() {
 }
 
 // FULL: every kernel variant (the tuned sizes); otherwise the defaults only (correlation
-// variant 6, fused-inverse variant 2), whatever the context asks for.
+// variant 6, fused-inverse variant 3 up to nf = 128 where it fits, else 2), whatever the
+// context asks for.
 template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     static_assert(pipe_size_ok<N>(), "no specialised pipeline for this size");
     constexpr int T = Geo<N>::T;
