@@ -11,6 +11,11 @@
  * device memory belongs to the context.  A context is used by one host thread at a
  * time (the reference binds OpenMP thread myid2 to device myid2,
  * src/fft_process.cpp:527).
+ *
+ * Ordering: mdk_dock_range queues work and returns; mdk_sync, mdk_get_results and
+ * mdk_topn wait for it.  The set-up calls (mdk_set_receptor*, mdk_set_ligand*,
+ * mdk_set_rotations) first wait for every queued batch of the context, so they may be
+ * called right after mdk_dock_range without mdk_sync.
  */
 #ifndef MDK_HIP_H
 #define MDK_HIP_H
@@ -44,8 +49,10 @@ int mdk_rotation_coef(const float *theta, int R, float *rotcoef);
 int mdk_device_count(int *count);
 
 /* Allocate device state for one receptor-ligand pair: grids of nf^3 points, ligand
- * voxel grids of ng^3, na_lig ligand atoms, t results per rotation (1..16) and
- * rotation batches of up to max_batch.
+ * voxel grids of ng^3, na_lig ligand atoms, t results per rotation (1..1048576; the
+ * reference's -t, src/parameter.cpp:101-102) and rotation batches of up to max_batch.
+ * Up to t = 16 the top-t lists are kept in registers by the inverse transform; above,
+ * each batch's score maps (max_batch * nf^3 floats) go through a per-rotation radix select.
  * Replaces FFTProcess::alloc_array / alloc_fft (src/fft_process.cpp:29-209). */
 int mdk_create(int device, int nf, int ng, int na_lig, int t, int max_batch, mdk_ctx **out);
 

@@ -29,6 +29,12 @@ int mdkh_protein_atoms(const mdkh_protein *p, float *xyz, float *radius, float *
  * size table `table` (-l: 11, 13, 7, 0 or 1): *ng = max(receptor ng, ligand ng). */
 int mdkh_autogrid(const mdkh_protein *rec, const mdkh_protein *lig, float gw, int table, int *ng);
 
+/* Built-in rotation set n (1, 3, 24, 360, 3600 or 54000, the -r flag): *count = n and,
+ * when angles is not NULL, the ZXZ Euler triples (3n floats, rad) as
+ * Parameter::parameter_set loads them into _Zangle (src/parameter.cpp:376-395,
+ * :3793-61886). */
+int mdkh_rotation_set(int n, float *angles, int *count);
+
 /* Grid coordinates, Docking::rec_init (src/docking.cpp:133-141). */
 int mdkh_grid_coord(int ng, float gw, float *out);
 

@@ -28,7 +28,7 @@ ABI_SYMBOLS = [
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
     "mdkh_protein_load", "mdkh_protein_free", "mdkh_protein_size", "mdkh_protein_atoms",
-    "mdkh_autogrid", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_run_table", "mdkh_last_error",
+    "mdkh_autogrid", "mdkh_rotation_set", "mdkh_grid_coord", "mdkh_receptor_voxel", "mdkh_run", "mdkh_run_table", "mdkh_last_error",
 ]
 STAGES = ["voxelize", "fwd_z", "fwd_y", "x_corr", "inv_y", "inv_z_top", "reduce"]
 # FFT sizes with compile-time specialised kernels (megadock_amd/csrc/mdk_pipe.hip): the
@@ -314,6 +314,15 @@ def autogrid(rec, lig, gw=1.2, table=11):
     ng = C.c_int(0)
     _hcheck(lib().mdkh_autogrid(rec.h, lig.h, C.c_float(gw), C.c_int(table), C.byref(ng)), "mdkh_autogrid")
     return ng.value
+
+
+def rotation_set(n):
+    """float32[n, 3]: built-in rotation set n as the host layer loads it (mdkh_rotation_set)."""
+    cnt = C.c_int(0)
+    _hcheck(lib().mdkh_rotation_set(C.c_int(n), None, C.byref(cnt)), "mdkh_rotation_set")
+    out = np.zeros((cnt.value, 3), np.float32)
+    _hcheck(lib().mdkh_rotation_set(C.c_int(n), _fp(out), C.byref(cnt)), "mdkh_rotation_set")
+    return out
 
 
 def grid_coord(ng, gw=1.2):

@@ -31,8 +31,13 @@ const float PI_F = (float)(4.0 * std::atan(1.0));  // src/constant.h:24
 
 int auto_batch(int nf, int ng, int t) {
     const double H = nf / 2 + 1;
-    const double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
-                                  (double)nf * nf * t);
+    double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
+                            (double)nf * nf * std::min(t, 16));
+    if (t > 16) {  // the score map of each rotation and its select keys (mdk_create)
+        double P = 1;
+        while (P < t) P *= 2;
+        per_rot += 4.0 * nf * nf * nf + 8.0 * P;
+    }
     // 8 GiB of batch buffers, at most 256 rotations a batch up to nf = 128 (184k against 175k
     // rotations/s at 96^3) and 128 up to nf = 192 (144^3: 61.8k for either, steadier at 128);
     // above, 36 GiB and at most 192 (256^3: 6.85k at 192 against 6.56k at 48 rotations)

@@ -87,6 +87,16 @@ int mdkh_autogrid(const mdkh_protein *rec, const mdkh_protein *lig, float gw, in
     return 0;
 }
 
+int mdkh_rotation_set(int n, float *angles, int *count) {
+    if (!count) return herr("mdkh_rotation_set: null count");
+    std::vector<float> a;
+    std::string err;
+    if (mdk::Parameter::load_rotation_set(n, a, err)) return herr(err);
+    *count = (int)(a.size() / 3);
+    if (angles) std::memcpy(angles, a.data(), sizeof(float) * a.size());
+    return 0;
+}
+
 int mdkh_grid_coord(int ng, float gw, float *out) {
     if (ng < 1 || !out) return herr("mdkh_grid_coord: invalid argument");
     const auto g = mdk::grid_coord(ng, gw);

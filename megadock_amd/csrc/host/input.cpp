@@ -153,7 +153,7 @@ int Parameter::initialize(int argc, char **argv, std::string &err) {
         csv_file = stem + ".csv";
     }
     std::printf("# Output file = %s\n", out_file.c_str());
-    if (num_sort < 1 || num_sort > 16) { err = "-t must be between 1 and 16 in this build"; return 1; }
+    if (num_sort < 1 || num_sort > 1048576) { err = "-t must be between 1 and 1048576"; return 1; }
     if (load_rotation_set(rotation_set, zangle, err)) return 1;
     num_rot_angles = (int)(zangle.size() / 3);
     if (num_rot_angles * num_sort < num_output) {  // src/parameter.cpp:396-400

@@ -436,6 +436,10 @@ struct BatchSlot {
     float2 *d_lg = nullptr, *d_a1 = nullptr, *d_a2 = nullptr, *d_cc = nullptr;
     Cand *d_cand = nullptr;
     mdk_sortscore *d_bout = nullptr;  // max_batch * t
+    // t > MDK_MAXT: the batch's score maps and the per-rotation select (launch_topt)
+    float *d_smap = nullptr;              // max_batch * nf^3
+    TopTState *d_tsel = nullptr;          // max_batch
+    unsigned long long *d_tkeys = nullptr;  // max_batch * topt_sort_size(t, nf)
     hipEvent_t done = nullptr;        // recorded after a range's last batch on this stream
     // one full batch captured as a graph (MDK_GRAPH): cursor, batch offset, executable
     int *d_ctl = nullptr;             // {base, step, count}, then the batch offset
@@ -515,7 +519,7 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
         a.nd = nd;
         a.H = c->H;
         a.Hp = c->Hp;
-        a.t = c->t;
+        a.t = smap_dev ? 1 : c->t;  // the score maps need no top-t lists
         a.nf3f = (float)((size_t)nf * nf * nf);
         a.vox = vox_grid(c, S);
         a.a1 = S.d_a1;
@@ -546,8 +550,9 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
     k_inv_y<<<dim3(c->Hp / WY, nf, B), NT, lds_bytes(c, WY), s>>>(S.d_cc, c->plan, c->d_tw, c->Hp);
     if (ev) HIPCHK(hipEventRecord(ev[3], s));
     const int rows = B * nf * nf;
-    const size_t lds_z = lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * c->t;
-    k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(S.d_cc, S.d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, c->t,
+    const int tl = smap_dev ? 1 : c->t;  // the score maps need no top-t lists
+    const size_t lds_z = lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * tl;
+    k_inv_z_top<<<(rows + WZ - 1) / WZ, NT, lds_z, s>>>(S.d_cc, S.d_cand, smap_dev, c->plan, c->d_tw, c->H, c->Hp, rows, tl,
                                                        (float)((size_t)nf * nf * nf));
     if (ev) HIPCHK(hipEventRecord(ev[4], s));
     HIPCHK(hipGetLastError());
@@ -561,8 +566,14 @@ int dock_batch(mdk_ctx *c, BatchSlot &S, const float *rot_dev, const int *id_dev
     if (ev) HIPCHK(hipEventRecord(ev[0], S.stream));
     HIPCHK(launch_voxelize(c->lig, rot_dev, B, S.vs, S.d_vidx, S.d_chg, S.stream, boff));
     if (ev) HIPCHK(hipEventRecord(ev[1], S.stream));
-    if (run_fft_pipeline(c, S, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int nf = c->nf;
+    if (c->t > MDK_MAXT) {  // score maps, then the per-rotation select
+        if (run_fft_pipeline(c, S, B, S.d_smap, ev ? ev + 2 : nullptr)) return 1;
+        HIPCHK(launch_topt(S.d_smap, B, nf, c->t, S.d_tsel, S.d_tkeys, out_dev, id_dev, boff, S.stream));
+        if (ev) HIPCHK(hipEventRecord(ev[7], S.stream));
+        return 0;
+    }
+    if (run_fft_pipeline(c, S, B, nullptr, ev ? ev + 2 : nullptr)) return 1;
     const int lists = c->use_pipe ? pipe_cands_per_rotation(nf) : nf * nf;
     k_reduce_top<<<B, NT, 0, S.stream>>>(S.d_cand, out_dev, id_dev, boff, lists, c->t, nf, (float)((size_t)nf * nf * nf));
     if (ev) HIPCHK(hipEventRecord(ev[7], S.stream));
@@ -602,6 +613,14 @@ void drop_graphs(mdk_ctx *c) {
             S.graph = nullptr;
         }
     }
+}
+
+// Set-up calls replace device data that queued batches read (ligand atoms, receptor
+// spectrum and grids, rotation table, results array): they first wait for every slot's
+// stream, so that a caller may change them right after an asynchronous mdk_dock_range.
+int wait_slots(mdk_ctx *c) {
+    for (int k = 0; k < c->nslots; k++) HIPCHK(hipStreamSynchronize(c->slot[k].stream));
+    return 0;
 }
 
 template <typename T> int dmalloc(T **p, size_t n) {
@@ -646,8 +665,8 @@ int mdk_device_count(int *count) {
 int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx **out) {
     if (!out) return fail("mdk_create: null output pointer");
     *out = nullptr;
-    if (nf < 4 || ng < 2 || ng > nf || na < 1 || t < 1 || t > MDK_MAXT || max_batch < 1)
-        return fail("mdk_create: invalid sizes (need 4 <= nf, 2 <= ng <= nf, na >= 1, 1 <= t <= 16, max_batch >= 1)");
+    if (nf < 4 || ng < 2 || ng > nf || na < 1 || t < 1 || t > MDK_MAXT_SELECT || max_batch < 1)
+        return fail("mdk_create: invalid sizes (need 4 <= nf, 2 <= ng <= nf, na >= 1, 1 <= t <= 1048576, max_batch >= 1)");
     if (nf != 2 * ng) return fail("mdk_create: nf must be 2*ng (src/docking.cpp:23-25)");
     if (na > 65535) return fail("mdk_create: at most 65535 ligand atoms");
     mdk_ctx *c = new mdk_ctx();
@@ -697,7 +716,11 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         BatchSlot &S = c->slot[k];
         if (dmalloc(&S.d_bits, B * vox_scratch_words(ng)) || dmalloc(&S.d_vidx, B * na) || dmalloc(&S.d_lg, B * ng3) ||
             dmalloc(&S.d_chg, B * ng3) || dmalloc(&S.d_a1, B * ng * ng * nf) || dmalloc(&S.d_a2, B * ng * nf * nf) ||
-            dmalloc(&S.d_cc, B * nf * nf * c->Hp) || dmalloc(&S.d_cand, B * nf * nf * t) || dmalloc(&S.d_bout, B * t))
+            dmalloc(&S.d_cc, B * nf * nf * c->Hp) || dmalloc(&S.d_cand, B * nf * nf * std::min(t, MDK_MAXT)) ||
+            dmalloc(&S.d_bout, B * t))
+            return bail(1);
+        if (t > MDK_MAXT && (dmalloc(&S.d_smap, B * nf3) || dmalloc(&S.d_tsel, B) ||
+                             dmalloc(&S.d_tkeys, B * topt_sort_size(t, nf))))
             return bail(1);
         S.vs = vox_scratch(S.d_bits, B, ng);
         // the atom counts and the charge grids start, and are kept between batches, all-zero
@@ -739,12 +762,13 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail("mdk_create: event creation failed"));
     // the largest kernel LDS request must fit
-    const size_t lds_max = std::max(lds_bytes(c, 2 * WX), lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * t);
+    const int tl = std::min(t, MDK_MAXT);  // above, the lists are not used (launch_topt)
+    const size_t lds_max = std::max(lds_bytes(c, 2 * WX), lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * tl);
     if (lds_max > 160 * 1024) return bail(fail("mdk_create: nf too large for the LDS-resident passes"));
     if (allow_lds(k_fwd_z, lds_bytes(c, WZ)) != hipSuccess || allow_lds(k_fwd_y, lds_bytes(c, WY)) != hipSuccess ||
         allow_lds(k_fwd_x_plain, lds_bytes(c, WY)) != hipSuccess ||
         allow_lds(k_x_corr, lds_bytes(c, 2 * WX)) != hipSuccess || allow_lds(k_inv_y, lds_bytes(c, WY)) != hipSuccess ||
-        allow_lds(k_inv_z_top, lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * t) != hipSuccess)
+        allow_lds(k_inv_z_top, lds_bytes(c, WZ) + sizeof(Cand) * (size_t)NT * tl) != hipSuccess)
         return bail(fail("mdk_create: cannot raise the LDS limit"));
     *out = c;
     return 0;
@@ -763,7 +787,8 @@ int mdk_destroy(mdk_ctx *c) {
     drop_graphs(c);
     for (int k = 0; k < mdk_ctx::kMaxSlots; k++) {
         BatchSlot &S = c->slot[k];
-        void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout, S.d_ctl};
+        void *sp[] = {S.d_bits, S.d_chg, S.d_vidx, S.d_lg, S.d_a1, S.d_a2, S.d_cc, S.d_cand, S.d_bout, S.d_ctl,
+                      S.d_smap, S.d_tsel, S.d_tkeys};
         for (void *p : sp)
             if (p) (void)hipFree(p);
         if (S.done) (void)hipEventDestroy(S.done);
@@ -794,6 +819,7 @@ int receptor_transform(mdk_ctx *c) {
 int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     if (!c || !grid_r || !grid_i) return fail("mdk_set_receptor: null argument");
     HIPCHK(hipSetDevice(c->device));
+    if (wait_slots(c)) return 1;
     const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
     std::vector<float2> h(ng3);
     for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
@@ -808,6 +834,7 @@ int mdk_set_receptor_atoms(mdk_ctx *c, int na, const float *xyz, const float *ra
         return fail("mdk_set_receptor_atoms: invalid argument");
     if (!(gw > 0.0f) || c->ng < 2) return fail("mdk_set_receptor_atoms: invalid grid");
     HIPCHK(hipSetDevice(c->device));
+    if (wait_slots(c)) return 1;
     const int ng = c->ng;
     const size_t n = na > 0 ? (size_t)na : 1;
     float *d_xyz = nullptr, *d_rad = nullptr, *d_chg = nullptr, *d_ace = nullptr, *d_gc = nullptr;
@@ -859,6 +886,7 @@ int mdk_get_receptor_grids(mdk_ctx *c, float *grid_r, float *grid_i) {
 int mdk_set_receptor_spectrum(mdk_ctx *c, const float *re, const float *im) {
     if (!c || !re || !im) return fail("mdk_set_receptor_spectrum: null argument");
     HIPCHK(hipSetDevice(c->device));
+    if (wait_slots(c)) return 1;
     const size_t nf3 = (size_t)c->nf * c->nf * c->nf;
     std::vector<float2> h(nf3);
     for (size_t i = 0; i < nf3; i++) h[i] = make_float2(re[i], im[i]);
@@ -895,6 +923,7 @@ int mdk_set_ligand_atoms(mdk_ctx *c, int na, const float *xyz, const float *cent
     if (delta == 0.0f) return fail("mdk_set_ligand: core value 0 is not supported (the reference's last shape sweep is order-dependent then)");
     if (!(gw > 0.0f)) return fail("mdk_set_ligand: grid width must be positive");
     HIPCHK(hipSetDevice(c->device));
+    if (wait_slots(c)) return 1;
     drop_graphs(c);  // they hold the ligand's kernel arguments
     std::vector<float> rc2(na), rs2(na);
     for (int i = 0; i < na; i++) {  // src/fft_process.cpp:784-787 / src/ligand.cpp:254-257
@@ -946,6 +975,7 @@ int mdk_dock(mdk_ctx *c, const float *rotcoef, const int *rot_id, int B, mdk_sor
 int mdk_set_rotations(mdk_ctx *c, const float *rotcoef, int R) {
     if (!c || !rotcoef || R < 1) return fail("mdk_set_rotations: invalid argument");
     HIPCHK(hipSetDevice(c->device));
+    if (wait_slots(c)) return 1;
     drop_graphs(c);  // they hold the old table and results pointers
     if (c->d_rot) (void)hipFree(c->d_rot);
     if (c->d_rotid) (void)hipFree(c->d_rotid);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mdk_hip.h"
+
 // Ligand data resident on the device (one context).
 struct LigandDev {
     int na;           // atoms
@@ -67,6 +69,24 @@ hipError_t launch_voxelize(const LigandDev &lig, const float *rot, int B, const 
 hipError_t launch_vox_expand(const VoxGrid &vg, int B, float2 *lg, hipStream_t s);
 
 #define MDK_MAXT 16  // largest t (results per rotation) the device top-t lists hold
+#define MDK_MAXT_SELECT (1 << 20)  // largest t of all: above MDK_MAXT, the score maps go through launch_topt
+
+// Per-rotation radix-select state of launch_topt (mdk_topt.hip)
+struct TopTState {
+    unsigned long long prefix, mask;  // key bits fixed so far
+    unsigned rank;                    // entries still wanted among those matching prefix
+    unsigned count;                   // compaction cursor
+    unsigned done;                    // the threshold is prefix (its lower bits zero)
+    unsigned pad;
+    unsigned hist[256];
+};
+// keys sorted per rotation by launch_topt for t results per rotation (a power of two)
+unsigned topt_sort_size(int t, int nf);
+// Top-t (t > MDK_MAXT) of B score maps smap[b][nf^3] (unnormalised real scores) into
+// out[(b0 + b) * t + q] as SortScore with rotation number rot_id[b0 + b], where b0 = *boff
+// (0 without boff); st: B states, keys: B * topt_sort_size(t, nf) words.
+hipError_t launch_topt(const float *smap, int B, int nf, int t, TopTState *st, unsigned long long *keys,
+                       mdk_sortscore *out, const int *rot_id, const int *boff, hipStream_t s);
 
 // A score candidate: its order key (see better()) and its linear grid index
 // (x*nf + y)*nf + z.

@@ -335,6 +335,17 @@ def ref_ligand_voxel(ng, gw, xyz_rot, delta=1.0, eratio=1.0):
     return gr, gi
 
 
+def ref_rotations(n, rec_pdb, lig_pdb):
+    """float32[n, 3]: the reference's own rotation table for -r n (Parameter::_Zangle after
+    Parameter::initialize, read through the HeaderDB friend in ref_driver.cpp)."""
+    ref_load(rec_pdb, lig_pdb, ["-r", str(n)])
+    r = ref()
+    cnt = r.ref_rotation_count()
+    out = np.zeros((cnt, 3), np.float32)
+    r.ref_rotation_angles(_fp(out))
+    return out
+
+
 def ref_rotate(xyz, theta):
     xyz = np.ascontiguousarray(xyz, np.float32)
     out = np.zeros_like(xyz)
@@ -357,11 +368,19 @@ GRID_TABLE_11 = [2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 18, 20, 21, 22,
                  64, 66, 70, 72, 75, 77, 80, 81, 84, 88, 90, 96, 98, 99, 100, 105, 108,
                  110, 112, 120, 121, 125, 126, 128, 132, 135, 140, 144, 147, 150,
                  154, 160, 162, 165, 168, 175, 176, 180, 189, 192, 196, 198, 200,
-                 210, 216, 220, 224, 225, 231, 240, 242, 243, 245, 250, 252, 256]
+                 210, 216, 220, 224, 225, 231, 240, 242, 243, 245, 250, 252, 256,
+                 264, 270, 275, 280, 288, 294, 297, 300, 308, 315, 320, 324, 330,
+                 336, 343, 350, 352, 360, 363, 375, 378, 384, 385, 392, 396, 400,
+                 405, 420, 432, 440, 441, 448, 450, 462, 480, 484, 486, 490, 495,
+                 500, 504, 512, 525, 528, 539, 540, 550, 560, 567, 576, 588, 594,
+                 600, 605, 616, 625, 630, 640, 648, 660, 672, 675, 686, 693, 700,
+                 704, 720, 726, 729, 735, 750, 756, 768, 770, 784, 792, 800, 810,
+                 825, 840, 847, 864, 875, 880, 882, 891, 896, 900, 924, 945, 960,
+                 968, 972, 980, 990, 1000, 1008, 1024]
 
 
 def autogrid(rec, lig, gw=1.2, table=GRID_TABLE_11):
-    """src/control.cpp:287-379 (11-base table, src/control.cpp:91-125, truncated at 256):
+    """src/control.cpp:287-379 (11-base table, src/control.cpp:91-125):
     returns ng = max(receptor ng, ligand ng)."""
     gw = np.float32(gw)
     edge_lo = rec.xyz.min(axis=0)

@@ -6,6 +6,8 @@
 // Receptor/Ligand voxelizers, its PDB reader and parameter tables, and its ZXZ rotation
 // (decoygen.cpp's rotateAtom, the same float expression as src/docking.cpp:376-393).
 // Nothing in the product links this file.
+#include <getopt.h>
+
 #include <cstring>
 #include <string>
 #include <vector>
@@ -15,6 +17,15 @@
 #include "ligand.h"
 
 void rotateAtom(float, float, float, float *, float *, float *, float, float, float);
+
+// src/parameter.h:39 names HeaderDB a friend of Parameter and the reference defines it
+// nowhere; defining it here gives the tests the rotation table exactly as
+// Parameter::parameter_set loads it (_Zangle, _Num_rot_angles: src/parameter.h:87-88).
+class HeaderDB {
+public:
+    static const float *angles(const Parameter *p) { return p->_Zangle; }
+    static int count(const Parameter *p) { return p->_Num_rot_angles; }
+};
 
 namespace {
 Parallel *g_par = nullptr;
@@ -45,6 +56,7 @@ int ref_load(const char *rec_pdb, const char *lig_pdb, int nextra, const char **
     delete g_rec; delete g_lig; delete g_prm; delete g_par;
     g_par = new Parallel(1);
     g_prm = new Parameter(g_par);
+    optind = 0;  // getopt restarts from argv[1] on every load (glibc: 0 reinitialises)
     g_prm->initialize((int)argv.size(), argv.data());
     g_rec = new Receptor(g_rec_file);
     g_rec->initialize(g_prm);
@@ -121,6 +133,14 @@ void ref_ligand_voxel(int ng, float gw, const float *xyz_rot, float aceratio, fl
     g_lig->electro(-2800.0f, eratio, ng, gc.data(), mol.data(), iw.data(), fw.data(), 0);
     std::memcpy(grid_r, fw.data(), sizeof(float) * ng3);
     std::memcpy(grid_i, fw.data() + ng3, sizeof(float) * ng3);
+}
+
+// the loaded rotation set (-r / -D): number of rotations, and 3 floats each into out
+int ref_rotation_count(void) { return HeaderDB::count(g_prm); }
+
+void ref_rotation_angles(float *out) {
+    const float *a = HeaderDB::angles(g_prm);
+    std::memcpy(out, a, sizeof(float) * 3 * (size_t)HeaderDB::count(g_prm));
 }
 
 // decoygen's rotateAtom: (x,y,z) rotated by ZXZ angles (a0, a1, a2)

@@ -27,7 +27,7 @@ CASES = [
     ("1gcq", "1gcq_r.pdb", "1gcq_l.pdb", 40, 1.0),
     ("1gcq_delta2", "1gcq_r.pdb", "1gcq_l.pdb", 40, 2.0),
     ("1gcq_small_grid", "1gcq_r.pdb", "1gcq_l.pdb", 24, 1.0),
-    ("synth_r120_l40", "synth_r120.pdb", "synth_l40.pdb", None, 1.0),
+    ("synth_r120_l40", "synth_r120.pdb", "synth_l40.pdb", 48, 1.0),
 ]
 
 
@@ -46,8 +46,6 @@ def main():
     for name, rf, lf, ng, delta in CASES:
         rp, lp = os.path.join(HERE, rf), os.path.join(HERE, lf)
         O.ref_load(rp, lp)
-        if ng is None:
-            ng = O.autogrid(O.Protein(rp), O.Protein(lp))
         rg = O.ref_receptor_voxel(ng, 1.2)
         refl = O.ref_protein(1)
         lig_xyz = refl["xyz"] - refl["center"]

@@ -198,37 +198,40 @@ def test_dock_vs_oracle(gpu, gcq, synth, kernels, t, which, ng):
     ctx.close()
 
 
+@pytest.mark.parametrize("t", [1, 2])
 @pytest.mark.parametrize("nf", M.FURTHER_NF)
-def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf):
+def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf, t):
     """The sizes specialised with the default kernel variants only (a table farm meets
     them: nf = 2 ng from the reference's FFT size table, including ng odd, where the
-    ligand block sits at floor(ng/2)): score map vs FP64 and per-rotation top-2 vs the
-    oracle, as for the tuned sizes."""
+    ligand block sits at floor(ng/2)): score map vs FP64 and per-rotation top-t vs the
+    oracle, as for the tuned sizes, for t = 1 (the reference's default: raw keys, a 0.0
+    starting value) and t = 2 (keys normalised by nf^3, a -99999 starting value)."""
     rec, lig = synth
     ng = nf // 2
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     rr, ri = O.receptor_fft(ng, nf, gr, gi)
-    ctx = _lig_ctx(lig, ng, t=2, max_batch=8)
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=8)
     ctx.set_receptor(gr, gi)
     ang = O.load_rotations(3600)
     remb = O.embed(ng, nf, gr, gi)
-    _say("further size nf=%d: score map" % nf)
-    a = 1234
-    smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
-    xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
-    lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
-    ref = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
-    assert np.abs(smap - ref).max() <= 2e-6 * np.abs(ref).max()
-    ids = np.arange(7, 3600, 900 if nf >= 160 else 450, dtype=np.int32)
-    _say("further size nf=%d: dock %d rotations" % (nf, len(ids)))
+    if t == 2:
+        _say("further size nf=%d: score map" % nf)
+        a = 1234
+        smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
+        xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+        lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+        ref = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+        assert np.abs(smap - ref).max() <= 2e-6 * np.abs(ref).max()
+    ids = np.arange(7 + t, 3600, 900 if nf >= 160 else 450, dtype=np.int32)
+    _say("further size nf=%d: dock %d rotations, t=%d" % (nf, len(ids), t))
     got = ctx.dock(M.rotation_coefs(ang[ids]), ids)
-    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, 2, threads=16)
-    for q in range(len(ids) * 2):
+    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, t, threads=16)
+    for q in range(len(ids) * t):
         g, r = got[q], exp[q]
         assert g["index"][0] == r["index"][0]
         assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
         if not np.array_equal(g["index"], r["index"]):
-            b = int(ids[q // 2])
+            b = int(ids[q // t])
             xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[b]))
             lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
             s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()

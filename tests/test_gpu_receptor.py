@@ -1,8 +1,11 @@
-"""GPU: receptor grids built on the device (mdk_set_receptor_atoms) are bit-exact with
-the host producer (mdkh_receptor_voxel, itself bit-exact with the reference's
-Receptor::rpscace/electro compiled in oracle/_ref), for several receptors, grids and the
--a/-d/-e/-v parameters, including cavities closed by the order-dependent propagation
-sweeps; the spectrum the device grids give equals mdk_set_receptor's."""
+"""GPU: receptor grids built on the device (mdk_set_receptor_atoms, read back with
+mdk_get_receptor_grids) are bit-exact with the oracle's restatement of the reference's
+Receptor::rpscace/electro (oracle/mdk_oracle.c, itself held bit for bit to the reference's
+own code compiled in oracle/_ref by tests/test_oracle_voxel.py and
+tests/test_host_parity.py) and with the host producer (mdkh_receptor_voxel), for several
+receptors, the c0, c1 and c4 workload receptors, grids and the -a/-d/-e/-v parameters,
+including cavities closed by the order-dependent propagation sweeps; the spectrum the
+device grids give equals mdk_set_receptor's."""
 import os
 
 import numpy as np
@@ -10,6 +13,7 @@ import pytest
 
 import megadock_amd as M
 from megadock_amd import workloads as W
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -22,6 +26,9 @@ def _check(prot, ng, gw=1.2, aceratio=1.0, rec_core=-45.0, eratio=1.0, lig=None)
     ctx.set_receptor_atoms(xyz, prot.radius, prot.charge, prot.ace, M.grid_coord(ng, gw), gw, aceratio, rec_core,
                            eratio)
     dr, di = ctx.receptor_grids()
+    orr, ori = O.receptor_voxel(ng, gw, O.Protein(prot.path), aceratio, rec_core, eratio)
+    assert np.array_equal(orr.view(np.uint32), dr.view(np.uint32)), "device vs oracle (shape)"
+    assert np.array_equal(ori.view(np.uint32), di.view(np.uint32)), "device vs oracle (electrostatics)"
     bad_r = int(np.count_nonzero(dr.view(np.uint32) != gr.view(np.uint32)))
     bad_i = int(np.count_nonzero(di.view(np.uint32) != gi.view(np.uint32)))
     print("ng=%d gw=%g ace=%g core=%g e=%g: %d / %d differing voxels; core %d, cavity-free open %d"
@@ -38,6 +45,13 @@ def _check(prot, ng, gw=1.2, aceratio=1.0, rec_core=-45.0, eratio=1.0, lig=None)
 def test_device_receptor_grids_are_bit_exact(golden_dir, which, ng, kw):
     prot = M.Protein(os.path.join(golden_dir, which + ".pdb"))
     ctx, gr, gi = _check(prot, ng, **kw)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,ng", [("c0", 48), ("c4", 128)])
+def test_device_receptor_grids_workloads(tmp_path, name, ng):
+    rec_pdb, lig_pdb, cfg = W.make_pair(name, str(tmp_path))
+    ctx, gr, gi = _check(M.Protein(rec_pdb), ng)
     ctx.close()
 
 

@@ -40,14 +40,14 @@ def test_golden_receptor_and_ligand_grids(golden, golden_dir):
             assert digest(li) == d["grid_i"], (name, a)
 
 
-def test_autogrid_matches_golden(golden, golden_dir):
-    case = golden["cases"]["synth_r120_l40"]
-    rec = O.Protein(os.path.join(golden_dir, case["receptor"]))
-    lig = O.Protein(os.path.join(golden_dir, case["ligand"]))
-    assert O.autogrid(rec, lig) == case["ng"]
-    rec = O.Protein(os.path.join(golden_dir, "1gcq_r.pdb"))
-    lig = O.Protein(os.path.join(golden_dir, "1gcq_l.pdb"))
-    assert O.autogrid(rec, lig) == 40  # ng=40, nf=80 for the reference's sample pair
+def test_autogrid_product_equals_oracle(golden_dir):
+    """Grid sizing of the golden pairs: the product's host layer and the oracle agree (both
+    are pinned to values derived by hand from src/control.cpp:287-379 in
+    tests/test_host_parity.py::test_autogrid_hand_derived)."""
+    import megadock_amd as M
+    for r, l in (("synth_r120.pdb", "synth_l40.pdb"), ("1gcq_r.pdb", "1gcq_l.pdb")):
+        rp, lp = os.path.join(golden_dir, r), os.path.join(golden_dir, l)
+        assert M.autogrid(M.Protein(rp), M.Protein(lp)) == O.autogrid(O.Protein(rp), O.Protein(lp))
 
 
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
