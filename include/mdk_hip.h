@@ -133,6 +133,16 @@ int mdk_get_results(mdk_ctx *ctx, int first, int count, mdk_sortscore *out);
  * FFTProcess::sort_index (src/fft_process.cpp:469-479), whose tie order is unspecified. */
 int mdk_topn(mdk_ctx *ctx, int nrot, int n, mdk_sortscore *out);
 
+/* As mdk_topn, allocating nothing: no hipMalloc, hipFree or allocating library call on
+ * any call.  No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops).  The merge state,
+ * keys and results live in one fixed array of 1 MiB per GPU (its address is looked up
+ * by mdk_create on each device and kept until the process ends), so min(n, nrot * t)
+ * may be at most 32768; larger requests fail with an error and mdk_topn serves them.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several host threads or contexts) race and must be serialised by the caller. */
+int mdk_topn_noalloc(mdk_ctx *ctx, int nrot, int n, mdk_sortscore *out);
+
 /* Time mdk_dock_range(first, count) `reps` times with HIP events on the context's
  * stream; *ms = mean milliseconds per repetition. */
 int mdk_time_range(mdk_ctx *ctx, int first, int count, int reps, float *ms);

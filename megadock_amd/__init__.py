@@ -23,7 +23,7 @@ ABI_SYMBOLS = [
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
     "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail", "mdk_set_receptor_atoms", "mdk_get_receptor_grids",
-    "mdk_decoy_coords",
+    "mdk_decoy_coords", "mdk_topn_noalloc",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [

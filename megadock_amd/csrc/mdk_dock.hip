@@ -28,11 +28,6 @@
 #include "mdk_internal.h"
 #include "../../include/mdk_hip.h"
 
-// device merge of the per-rotation results (mdk_merge.hip)
-size_t topn_scratch_bytes(int n);
-hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *scratch, mdk_sortscore *out,
-                       hipStream_t s);
-
 // real-space detail sums (mdk_detail.hip)
 size_t detail_scratch_doubles(int max_batch, int ng);
 hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
@@ -690,6 +685,14 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
     };
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail("mdk_create: hipSetDevice failed"); }
     {
+        // the fixed merge array of mdk_topn_noalloc: its address on this device, once per device
+        const hipError_t e = topn_static_setup(device);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(std::string("mdk_create: hipGetSymbolAddress of the merge array failed: ") + hipGetErrorString(e));
+        }
+    }
+    {
         // two batches in flight on two streams (one kernel's tail overlaps the next batch's
         // first kernels; 61.0k against 59.6k rotations/s at 144^3; a third stream gave
         // 61.5k-62.8k against 63.0k in full runs), unless the batches are small or
@@ -1061,6 +1064,26 @@ int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
     }
     HIPCHK(launch_topn(c->d_results, (unsigned)m, (unsigned)keep, c->d_topn_scratch, c->d_topn_out, c->stream));
     HIPCHK(hipMemcpyAsync(out, c->d_topn_out, sizeof(mdk_sortscore) * keep, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mdk_topn_noalloc(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
+    if (!c || !out || !c->d_results) return fail("mdk_topn_noalloc: nothing to merge");
+    if (nrot < 1 || nrot > c->nrot || n < 1) return fail("mdk_topn_noalloc: invalid count");
+    const size_t m = (size_t)nrot * c->t;
+    const int keep = (int)std::min((size_t)n, m);
+    if (keep > MDK_TOPN_STATIC_MAX)
+        return fail("mdk_topn_noalloc: " + std::to_string(keep) + " results exceed the fixed merge array's maximum of " +
+                    std::to_string(MDK_TOPN_STATIC_MAX) + " (use mdk_topn)");
+    if (m > 0xffffffffull) return fail("mdk_topn_noalloc: too many results to merge");
+    HIPCHK(hipSetDevice(c->device));
+    char *base = static_cast<char *>(topn_static_ptr(c->device));  // looked up by mdk_create
+    if (!base) return fail("mdk_topn_noalloc: the merge array of this device was not looked up");
+    mdk_sortscore *dout = reinterpret_cast<mdk_sortscore *>(base + topn_static_out_offset(keep));
+    // the same kernels as mdk_topn; they initialise every word of the array they read
+    HIPCHK(launch_topn(c->d_results, (unsigned)m, (unsigned)keep, base, dout, c->stream));
+    HIPCHK(hipMemcpyAsync(out, dout, sizeof(mdk_sortscore) * keep, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

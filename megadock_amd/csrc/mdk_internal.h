@@ -291,3 +291,19 @@ bool pipe_available(int nf);
 // top-t candidate lists the specialised pipeline leaves per rotation (each of t entries)
 int pipe_cands_per_rotation(int nf);
 hipError_t pipe_run(int nf, const PipeArgs &a);
+
+// Device merge (mdk_merge.hip): scratch bytes of launch_topn for n results, and the merge of
+// the best n of m results res[] into out[] (work: topn_scratch_bytes(n) bytes of device memory)
+size_t topn_scratch_bytes(int n);
+hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *work, mdk_sortscore *out,
+                       hipStream_t s);
+// The allocation-free merge (mdk_topn_noalloc) keeps its select state, keys and n results
+// in a fixed device array of MDK_TOPN_STATIC_BYTES per GPU: n <= MDK_TOPN_STATIC_MAX fits.
+#define MDK_TOPN_STATIC_BYTES (1u << 20)
+#define MDK_TOPN_STATIC_MAX 32768
+// byte offset of the results within that array for n results
+size_t topn_static_out_offset(int n);
+// look the array's device address up for `device` (which must be current); once per device
+hipError_t topn_static_setup(int device);
+// the address looked up by topn_static_setup, or nullptr
+void *topn_static_ptr(int device);

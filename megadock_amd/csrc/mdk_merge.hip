@@ -14,6 +14,8 @@
 //   4. gather of the results in key order.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mdk_internal.h"
 #include "../../include/mdk_hip.h"
 
@@ -142,15 +144,15 @@ size_t topn_scratch_bytes(int n) {
     return sizeof(SelState) + sizeof(unsigned long long) * P + 256;
 }
 
-hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *scratch, mdk_sortscore *out,
+hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *work, mdk_sortscore *out,
                        hipStream_t s) {
     if (n == 0 || m == 0) return hipSuccess;
     if (n > m) n = m;
     unsigned P = 1;
     while (P < n) P <<= 1;
-    SelState *st = reinterpret_cast<SelState *>(scratch);
+    SelState *st = reinterpret_cast<SelState *>(work);
     unsigned long long *keys =
-        reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(scratch) + ((sizeof(SelState) + 255) & ~size_t(255)));
+        reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(work) + ((sizeof(SelState) + 255) & ~size_t(255)));
     const unsigned blocks = (m + 255u) / 256u < 1024u ? (m + 255u) / 256u : 1024u;
     k_sel_init<<<1, 256, 0, s>>>(st, n);
     for (int shift = 56; shift >= 0; shift -= 8) {
@@ -167,4 +169,43 @@ hipError_t launch_topn(const mdk_sortscore *res, unsigned m, unsigned n, void *s
     }
     k_sel_gather<<<(n + 255u) / 256u, 256, 0, s>>>(keys, n, res, out);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ allocation-free merge
+// Fixed device scratch of mdk_topn_noalloc: the select state and keys of launch_topn,
+// then the n merged results.  Every GPU the library runs on holds its own copy until the
+// process ends.  Host code never uses the array's name as an address: the address of
+// this device's copy comes from hipGetSymbolAddress, looked up once per device at
+// context creation (topn_static_setup) and kept in g_static_ptr.
+static __device__ __align__(256) unsigned char scratch[MDK_TOPN_STATIC_BYTES];
+// the largest merge fits: state, keys of the power of two MDK_TOPN_STATIC_MAX, then results
+static_assert((MDK_TOPN_STATIC_MAX & (MDK_TOPN_STATIC_MAX - 1)) == 0, "the key count is a power of two");
+static_assert(((sizeof(SelState) + 8ull * MDK_TOPN_STATIC_MAX + 256 + 255) & ~255ull) +
+                      sizeof(mdk_sortscore) * MDK_TOPN_STATIC_MAX <= MDK_TOPN_STATIC_BYTES,
+              "the fixed merge array holds MDK_TOPN_STATIC_MAX results");
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[kMaxDevices] = {};
+}  // namespace
+
+size_t topn_static_out_offset(int n) { return (topn_scratch_bytes(n) + 255) & ~size_t(255); }
+
+hipError_t topn_static_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (g_static_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // this device is current
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_static_ptr[device] = p;
+    return hipSuccess;
+}
+
+void *topn_static_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    return g_static_ptr[device];
 }
