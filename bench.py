@@ -36,20 +36,42 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def stage_bytes(nf, ng, t):
-    """Algorithmic HBM bytes per rotation of each stage (its inputs read once and its
-    outputs written once)."""
-    H = nf // 2 + 1
-    c = 8.0  # complex64
-    return {
-        "voxelize": c * ng ** 3,
-        "fwd_z": c * (ng ** 3 + ng * ng * nf),
-        "fwd_y": c * (ng * ng * nf + ng * nf * nf),
-        "x_corr": c * (ng * nf * nf + nf ** 3 + nf * nf * H),
-        "inv_y": c * 2 * nf * nf * H,
-        "inv_z_top": c * nf * nf * H,
-        "reduce": 8.0 * nf * nf * t,
-    }
+# Algorithmic HBM bytes per rotation, B_rot = 32 nf^3: the forward and the inverse 3D FFT
+# each read and write the complex64 grid once (4 G, G = 8 nf^3).  Each of the six axis
+# passes (z, y, x forward; x, y, z inverse) carries a sixth of it; voxelization, the
+# spectral product (fused into the x passes) and the top-t reduction carry none.  A kernel
+# that performs several passes carries their sum, so the shares of the kernels of one batch
+# add up to B_rot.
+STAGE_PASSES = {"voxelize": 0, "fwd_z": 1, "fwd_y": 1, "x_corr": 2, "inv_y": 1, "inv_z_top": 1, "reduce": 0}
+STAGE_KERNELS = {"voxelize": "k_vox_lds / k_vox_atoms + k_vox_masks, k_vox_charge", "fwd_z": "kp_fwd_z",
+                 "fwd_y": "kp_fwd_y", "x_corr": "kp_x_corr", "inv_y": "kp_inv_y", "inv_z_top": "kp_inv_z_top",
+                 "reduce": "k_reduce_top"}
+VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions per second: one per CU per cycle, 256 CUs at 2.4 GHz
+
+
+def kernel_table(stage_ms, nfull, nf, batch):
+    """Per-kernel launch durations (HIP events around each stage of every full batch of the
+    timed passes) with their algorithmic bytes.  A stage whose kernel is fused into a
+    neighbour has an empty event interval: its passes move to that neighbour (forward y
+    into the fused forward z + y kernel, inverse y into the fused inverse y + z + top-t)."""
+    b_rot = 32.0 * nf ** 3
+    ms = {k: v / max(nfull, 1) for k, v in stage_ms.items()}
+    passes = dict(STAGE_PASSES)
+    names = dict(STAGE_KERNELS)
+    empty = lambda k: ms[k] < 0.02 * max(ms.values())
+    if empty("fwd_y"):
+        passes["fwd_z"] += passes.pop("fwd_y")
+        names["fwd_z"] = "kp_fwd_zy (forward z + y)"
+    if empty("inv_y"):
+        passes["inv_z_top"] += passes.pop("inv_y")
+        names["inv_z_top"] = "kp_inv_yz_top (inverse y + z + top-t)"
+    rows = {}
+    for k in passes:
+        share = passes[k] / 6.0
+        rows[k] = {"kernel": names[k], "ms_per_launch": round(ms[k], 4), "B_rot_share": round(share, 4),
+                   "alg_bytes_per_launch": share * b_rot * batch,
+                   "GBps": round(share * b_rot * batch / (ms[k] * 1e-3) / 1e9, 1) if share > 0 else None}
+    return rows
 
 
 def rocfft_reference(nf, batch, reps=5):
@@ -227,6 +249,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT (hipFFT) reference point")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy 1 GiB device to device (calibrates the HBM counters of a profiled run)")
     ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "mdk_workloads"))
     ap.add_argument("--sweep", default="", help="tuning: comma list of correlation-pass variants (v), fused-inverse variants (i), "
                     "stream counts (s), graph replay on/off (g), forward-pass variants (f) and batch sizes (b), e.g. 'v0,v3,i1,s1,b64'; settings accumulate; prints "
@@ -242,6 +266,10 @@ def main():
 
     import megadock_amd as M
     from megadock_amd import workloads
+
+    if args.copy_probe:
+        cms = M.copy_probe(1 << 30, reps=2)
+        log("copy probe: 1 GiB in %.3f ms" % cms)
 
     if args.workload == "c2":
         return bench_c2(args, M, workloads)
@@ -305,29 +333,53 @@ def main():
                               "stages_ms": {k: round(v, 4) for k, v in st.items()}}), flush=True)
             sctx.close()
         return
+    n_out = min(cfg["n_out"], R * t)
     for w in range(args.warmup):
         ctx.dock_range(0, R)
-        ctx.sync()
+        ctx.topn(R, n_out)
         log("warmup pass %d done" % (w + 1))
     ctx.sync()
+    # A step: every rotation docked (batches on the context's streams, HIP events around each
+    # stage of each batch), then the merge of the best n_out decoys on the device.
+    stage_ms = {k: 0.0 for k in M.STAGES}
+    nfull = 0
     t0 = time.perf_counter()
     for s in range(args.steps):
-        ctx.dock_range(0, R)
-    ctx.sync()
+        st, nb = ctx.dock_range_timed(0, R)
+        top = ctx.topn(R, n_out)
+        for k in st:
+            stage_ms[k] += st[k]
+        nfull += nb
     wall = time.perf_counter() - t0
-    ev_ms = ctx.time_range(0, R, reps=1)  # one more pass timed with HIP events on the stream
-    log("timed %d passes: %.3f s wall, %.1f ms per pass by events" % (args.steps, wall, ev_ms))
+    ev_ms = ctx.time_range(0, R, reps=1)  # one pass without the stage events, timed by events on the stream
+    log("timed %d steps: %.3f s wall; one pass without stage events %.1f ms" % (args.steps, wall, ev_ms))
     rot_s = args.steps * R / wall
-    top = ctx.topn(R, min(cfg["n_out"], R * t))
+    nb_rot = min(B, R)
+    kernels = kernel_table(stage_ms, nfull, nf, nb_rot)
+    dominant = max((k for k in kernels if kernels[k]["B_rot_share"] > 0), key=lambda k: kernels[k]["ms_per_launch"])
+    dk = kernels[dominant]
 
-    # per-stage breakdown of one full batch
-    stages = ctx.profile_stages(0, min(B, R))
-    sbytes = stage_bytes(nf, ng, t)
-    nb = min(B, R)
-    kernels = {k: {"ms_per_batch": round(v, 4), "GBps": round(sbytes[k] * nb / (v * 1e-3) / 1e9, 1)}
-               for k, v in stages.items()}
-    dominant = max(stages, key=stages.get)
-    batch_ms = sum(stages.values())
+    # the same kernels with one batch in flight (one stream): each kernel's duration is then
+    # its own, not shared with the other stream's kernels (reported beside, not timed above)
+    env = {"MDK_STREAMS": "1"}
+    if tuned:  # the variants the timed passes ran
+        env.update(MDK_XCORR_VARIANT=str(tuned[0]), MDK_INV_VARIANT=str(tuned[1]))
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    octx = M.Context(nf, ng, lig.na, t=t, max_batch=B)
+    for k, v in saved.items():
+        if v is None:
+            os.environ.pop(k)
+        else:
+            os.environ[k] = v
+    octx.set_receptor(gr, gi)
+    octx.set_ligand(lig.xyz, lig.center, lig.radius, lig.charge, M.grid_coord(ng), 1.2)
+    octx.set_rotations(rot[:min(R, 16 * B)])
+    octx.dock_range(0, min(R, 2 * B))
+    octx.sync()
+    ost, onb = octx.dock_range_timed(0, min(R, 16 * B))
+    octx.close()
+    one_stream = {k: v for k, v in kernel_table(ost, onb, nf, nb_rot).items()}
 
     # plain path: one rotation per call, host round trip each
     P = min(args.plain_rotations, R)
@@ -346,18 +398,27 @@ def main():
                      for q in range(P))
 
     b_rot = 32.0 * nf ** 3
-    achieved = rot_s * b_rot / 1e9
-    # measured HBM bytes per rotation of the same pipeline (rocprofv3 FETCH_SIZE/WRITE_SIZE,
-    # copy-calibrated; written by `python -m megadock_amd.profiling traffic`)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_%s_nf%d.json" % (args.workload, nf))
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            tr = json.load(fh)
-        traffic = {"bytes_per_batch": round(tr["bytes_per_rotation"] * min(B, R)),
-                   "bytes_per_rotation": round(tr["bytes_per_rotation"]),
-                   "GBps_at_this_rate": round(tr["bytes_per_rotation"] * rot_s / 1e9, 1),
-                   "source": os.path.relpath(tfile, ROOT)}
+    achieved = dk["alg_bytes_per_launch"] / (dk["ms_per_launch"] * 1e-3) / 1e9
+    # measured HBM bytes and VALU instructions per launch (rocprofv3 --pmc runs of this
+    # command, summarised by `python -m megadock_amd.profiling`, committed under profiles/)
+    prof = None
+    pfile = os.path.join(ROOT, "profiles", "pmc_%s_nf%d.json" % (args.workload, nf))
+    if os.path.exists(pfile):
+        with open(pfile) as fh:
+            prof = json.load(fh)
+
+    def pmc(stage, key):
+        if not prof:
+            return None
+        hit = [v for k, v in prof["kernels"].items() if k.startswith(kernels[stage]["kernel"].split(" ")[0] + "<")
+               or k == kernels[stage]["kernel"].split(" ")[0]]
+        return sum(h[key] for h in hit) / prof["batch"] * nb_rot if hit and all(key in h for h in hit) else None
+
+    traffic = pmc(dominant, "hbm_bytes")
+    valu = pmc(dominant, "SQ_INSTS_VALU")
+    pipe_traffic = None
+    if prof and "pipeline_hbm_bytes" in prof:
+        pipe_traffic = prof["pipeline_hbm_bytes"] / prof["batch"]
     out = {
         "metric": METRIC,
         "value": round(rot_s, 1),
@@ -374,20 +435,32 @@ def main():
         "config": {"workload": "%s: synthetic receptor %d res (%d atoms) / ligand %d res (%d atoms), %d^3 FP32 complex grid, "
                                "%d rotations, top %d per rotation, %d decoys"
                                % (args.workload, cfg["rec_res"], rec.na, cfg["lig_res"], lig.na, nf, R, t, cfg["n_out"]),
-                   "nf": nf, "ng": ng, "rotations": R, "t": t, "batch": B,
+                   "nf": nf, "ng": ng, "rotations": R, "t": t, "decoys": n_out, "batch": B,
+                   "step": "dock all rotations + device merge of the best decoys",
                    "variants": {"correlation": tuned[0], "inverse": tuned[1], "chosen_by": "autotune"} if tuned
                    else "defaults"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic["bytes_per_batch"] if traffic else None,
-                     "traffic_detail": traffic,
-                     "unit_of_work": "one batch of %d rotations (the per-batch pipeline of %d kernels on one stream)"
-                                     % (nb, 10),
-                     "bytes_per_rotation": b_rot, "dominant_kernel": dominant,
-                     "batch_ms_events": round(batch_ms, 4), "kernels": kernels},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(traffic) if traffic else None,
+            "kernel": dk["kernel"], "launches": nfull, "rotations_per_launch": nb_rot,
+            "avg_launch_ms": dk["ms_per_launch"],
+            "alg_bytes_per_launch": dk["alg_bytes_per_launch"], "B_rot_share": dk["B_rot_share"],
+            "measured_hbm_frac": round(traffic / (dk["ms_per_launch"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4) if traffic else None,
+            "limiter": ({"what": "VALU instruction issue", "valu_instructions_per_launch": round(valu),
+                         "valu_issue_frac": round(valu / (dk["ms_per_launch"] * 1e-3) / VALU_ISSUE_PEAK, 4)}
+                        if valu else None),
+            "counters_source": os.path.relpath(pfile, ROOT) if prof else None,
+            "pipeline": {"B_rot": b_rot, "achieved": round(rot_s * b_rot / 1e9, 1),
+                         "frac": round(rot_s * b_rot / 1e9 / HBM_PEAK_GBS, 4),
+                         "measured_hbm_bytes_per_rotation": round(pipe_traffic) if pipe_traffic else None,
+                         "measured_hbm_frac": round(rot_s * pipe_traffic / 1e9 / HBM_PEAK_GBS, 4) if pipe_traffic else None},
+            "kernels": kernels,
+            "kernels_one_stream": {"note": "%d full batches on one stream, outside the timed region: each "
+                                           "kernel's own duration" % onb, "kernels": one_stream}},
         "plain": {"value": round(plain_rot_s, 1), "unit": "rotations/s", "rotations": P,
                   "batched_over_plain": round(rot_s / plain_rot_s, 2), "bitwise_equal_to_batched": bool(plain_same)},
-        "events_ms_per_pass": round(ev_ms, 2),
+        "pass_ms_without_stage_events": round(ev_ms, 2),
         "best_score": float(top["score"][0]),
     }
 
@@ -396,8 +469,17 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        n_cpu = args.cpu_rotations or {"c0": 512, "c1": 256, "c4": 24}[args.workload]
+        # the host cores this job may use: its CPU affinity, capped by OMP_NUM_THREADS when the
+        # machine sets it (a one-GPU share of a shared host)
+        affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = min(affinity, int(os.environ.get("OMP_NUM_THREADS", "0")) or affinity)
+        model = None
+        try:
+            with open("/proc/cpuinfo") as fh:
+                model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), None)
+        except OSError:
+            pass
+        n_cpu = args.cpu_rotations or {"c0": 2048, "c1": 768, "c4": 48}[args.workload]
         orec, olig = O.Protein(rec_pdb), O.Protein(lig_pdb)
         ogr, ogi = O.receptor_voxel(ng, 1.2, orec)
         rr, ri = O.receptor_fft(ng, nf, ogr, ogi)
@@ -411,10 +493,12 @@ def main():
         rel = np.abs(g["score"].astype(np.float64) - c["score"]) / np.maximum(np.abs(c["score"]), 1e-3)
         out["cpu_baseline"] = {"value": round(n_cpu / cpu_s, 2), "unit": "rotations/s", "cores": threads,
                                "kind": "port",
-                               "sample": "%d of the %d rotations (every %d-th), full %d^3 loop per rotation; "
-                                         "oracle's own mixed-radix FFT (FFTW not installed)"
-                                         % (n_cpu, R, R // n_cpu, nf),
-                               "seconds": round(cpu_s, 2)}
+                               "sample": "%d of the %d rotations (every %d-th), full %d^3 loop per rotation (voxelize, "
+                                         "forward FFT, product, inverse FFT, top-t) on OpenMP threads; the oracle's own "
+                                         "mixed-radix FFT (FFTW not installed)" % (n_cpu, R, R // n_cpu, nf),
+                               "seconds": round(cpu_s, 2),
+                               "host": {"nproc": os.cpu_count(), "affinity_cpus": affinity,
+                                        "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "model": model}}
         # the global merge over the sample: the n best of its n_cpu * t results, as sets
         n_top = max(1, n_cpu * t // 4)
 

@@ -114,6 +114,14 @@ int mdk_set_ligand_atoms(mdk_ctx *ctx, int na, const float *xyz, const float *ce
  * (src/fft_process.cpp:502-765), batched. */
 int mdk_dock(mdk_ctx *ctx, const float *rotcoef, const int *rot_id, int B, mdk_sortscore *out);
 
+/* Dock one rotation whose ligand grids were voxelized by the caller: grid_r (shape) and
+ * grid_i (charge), ng^3 floats each, row-major x slowest, as Ligand::rpscace and
+ * Ligand::electro leave them (src/ligand.cpp:20-239); writes the rotation's t results
+ * with index[0] = rot_id.  Replaces the CPU branch of the reference's loop,
+ * FFTProcess::ligand_preparation + fft3d + convolution + score_sort
+ * (src/fft_process.cpp:306-458), on the device. */
+int mdk_dock_grid(mdk_ctx *ctx, const float *grid_r, const float *grid_i, int rot_id, mdk_sortscore *out);
+
 /* Device-resident rotation table (R*9 coefficients, rotation i has number i), so that
  * mdk_dock_range runs without host traffic (the _Zangle table, src/parameter.h:87). */
 int mdk_set_rotations(mdk_ctx *ctx, const float *rotcoef, int R);
@@ -152,6 +160,13 @@ int mdk_time_range(mdk_ctx *ctx, int first, int count, int reps, float *ms);
  * x + correlation, inverse y, inverse z + top-t, per-rotation reduce); ms[q] gets each
  * stage's milliseconds and *nstages their number (7). No reference counterpart. */
 int mdk_profile_stages(mdk_ctx *ctx, int first, int count, float *ms, int *nstages);
+
+/* Measurement hook: mdk_dock_range(first, count) with HIP events recorded on each
+ * batch's stream around each of its stages (as mdk_profile_stages), then a wait for the
+ * whole range; stage_ms[q] gets the summed milliseconds of stage q over the range's full
+ * batches (*nfull of them; a last, partial batch is left out) and *nstages their number
+ * (7).  Results are those of mdk_dock_range.  No reference counterpart. */
+int mdk_dock_range_timed(mdk_ctx *ctx, int first, int count, float *stage_ms, int *nstages, int *nfull);
 
 /* Test hook: ligand voxel grids for B rotations (B*ng^3 floats each). */
 int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float *grid_i);
@@ -193,14 +208,17 @@ int mdk_score_companion(mdk_ctx *ctx, const float *rotcoef, const float *t, int 
  * and grid translations trans (B*3, a decoy's index[1..3], within -ng .. 3ng-1).  The
  * ligand is voxelized on the device; rec_rpsc (ng^3) is the receptor shape grid built
  * without the ACE term (aceratio 0), the other receptor grids are mdk_set_receptor's.
- * out (B*6, double): rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-), the float
- * products of the reference summed in double.  Replaces the voxel loops of
- * src/docking.cpp:596-657 and :710-801. */
-int mdk_score_detail(mdk_ctx *ctx, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, double *out);
+ * out (B*9): rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-) (the decoys' sums,
+ * src/docking.cpp:746-807), then rPSC, ELEC, ACE unsplit (the sums of the poses around the
+ * input position, :646-654); each the float sum of the reference's float products in its
+ * order (l, m, n ascending), so equal to the reference's bit for bit.  Replaces the voxel
+ * loops of src/docking.cpp:596-657 and :710-801. */
+int mdk_score_detail(mdk_ctx *ctx, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, float *out);
 
-/* Measurement hook: device-to-device copy of `bytes` (read + write), `reps` times;
- * *ms = mean milliseconds per copy.  Calibrates HBM counters and the achievable copy
- * bandwidth.  No reference counterpart. */
+/* Measurement hook: device-to-device copy of `bytes` (a multiple of 16; read + write) by
+ * one dispatch of the kernel k_copy_probe, `reps` times after one untimed copy; *ms = mean
+ * milliseconds per copy.  Calibrates HBM counters (a profiler sees each copy's bytes in
+ * one dispatch) and the achievable copy bandwidth.  No reference counterpart. */
 int mdk_copy_probe(int device, size_t bytes, int reps, float *ms);
 
 /* Test hook: `count` independent n-point complex FFTs (interleaved complex64), sign

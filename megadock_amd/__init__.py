@@ -23,7 +23,7 @@ ABI_SYMBOLS = [
     "mdk_get_results", "mdk_topn", "mdk_time_range", "mdk_profile_stages", "mdk_voxelize",
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
     "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail", "mdk_set_receptor_atoms", "mdk_get_receptor_grids",
-    "mdk_decoy_coords", "mdk_topn_noalloc",
+    "mdk_decoy_coords", "mdk_topn_noalloc", "mdk_dock_range_timed", "mdk_dock_grid",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -189,6 +189,15 @@ class Context:
         _check(lib().mdk_topn(self.h, C.c_int(nrot), C.c_int(n), C.c_void_p(out.ctypes.data)), "mdk_topn")
         return out
 
+    def dock_range_timed(self, first, count):
+        """dock_range + sync with HIP events around every stage of every batch: returns
+        ({stage: summed ms over the full batches}, number of full batches)."""
+        ms = (C.c_float * 16)()
+        ns, nfull = C.c_int(0), C.c_int(0)
+        _check(lib().mdk_dock_range_timed(self.h, C.c_int(first), C.c_int(count), ms, C.byref(ns), C.byref(nfull)),
+               "mdk_dock_range_timed")
+        return {STAGES[q]: ms[q] for q in range(ns.value)}, nfull.value
+
     def time_range(self, first, count, reps=1):
         ms = C.c_float(0.0)
         _check(lib().mdk_time_range(self.h, C.c_int(first), C.c_int(count), C.c_int(reps), C.byref(ms)),
@@ -228,16 +237,17 @@ class Context:
         return out
 
     def score_detail(self, rec_rpsc, rotcoef, trans):
-        """(B, 6) float64: real-space rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-) of
-        poses at grid translations trans (B, 3) int, as the reference's detailed output
-        sums them; rec_rpsc is the receptor shape grid built without the ACE term."""
+        """(B, 9) float32: real-space rPSC(+), rPSC(-), ELEC(+), ELEC(-), ACE(+), ACE(-), then
+        rPSC, ELEC, ACE unsplit, of poses at grid translations trans (B, 3) int, the float
+        sums of the reference's detailed output in its order; rec_rpsc is the receptor shape
+        grid built without the ACE term."""
         rp = np.ascontiguousarray(rec_rpsc, np.float32).ravel()
         assert rp.size == self.ng ** 3
         rc = np.ascontiguousarray(rotcoef, np.float32).reshape(-1, 9)
         tr = np.ascontiguousarray(trans, np.int32).reshape(-1, 3)
-        out = np.zeros((len(rc), 6), np.float64)
+        out = np.zeros((len(rc), 9), np.float32)
         _check(lib().mdk_score_detail(self.h, _fp(rp), _fp(rc), tr.ctypes.data_as(C.POINTER(C.c_int)),
-                                      C.c_int(len(rc)), out.ctypes.data_as(C.POINTER(C.c_double))),
+                                      C.c_int(len(rc)), _fp(out)),
                "mdk_score_detail")
         return out
 

@@ -255,7 +255,7 @@ int write_detail(const Parameter &prm, const Receptor &rec, const Ligand &lig, i
         err = std::string("mdk_rotation_coef: ") + mdk_last_error();
         return 1;
     }
-    std::vector<double> sums(6 * (size_t)np);
+    std::vector<float> sums(9 * (size_t)np);  // the reference's float sums, in its order
     if (mdk_score_detail(ctx, rpsc.data(), rot.data(), trans.data(), np, sums.data())) {
         err = std::string("mdk_score_detail: ") + mdk_last_error();
         return 1;
@@ -270,12 +270,15 @@ int write_detail(const Parameter &prm, const Receptor &rec, const Ligand &lig, i
     }
     std::vector<DetailRow> row(np);
     for (int p = 0; p < np; p++) {
-        const double *d = &sums[6 * (size_t)p];
+        const float *d = &sums[9 * (size_t)p];
         DetailRow &r = row[p];
-        r.rpsc_plus = (float)d[0];
-        r.rpsc_minus = (float)d[1];
-        r.rpsc = r.rpsc_plus + r.rpsc_minus;
-        const float elec = (float)d[2] + (float)d[3], ace = (float)d[4] + (float)d[5];
+        r.rpsc_plus = d[0];
+        r.rpsc_minus = d[1];
+        // the poses around the input position print the unsplit sums (src/docking.cpp:646-655),
+        // the decoys the sums of their split parts (:803-807)
+        const bool around = p < 7;
+        r.rpsc = around ? d[6] : r.rpsc_plus + r.rpsc_minus;
+        const float elec = around ? d[7] : d[2] + d[3], ace = around ? d[8] : d[4] + d[5];
         r.elec = elec;
         r.ace = ace;
         r.total = r.rpsc + r.elec + r.ace;

@@ -13,10 +13,15 @@
 //     PDB reading, grid sizing, the receptor's shape/electrostatics grids (cached per
 //     receptor file and grid, since a table repeats receptors), rotation coefficients;
 //   the calling thread docks the prepared jobs on GPU 0 through libmdk contexts kept per
-//     FFT size (re-created only when a job needs more ligand atoms, another t or another
-//     batch; at most six alive, the least recently used freed first), taking first a
-//     prepared job whose size has a live context, and writes each job's .out file while
-//     the next job's batches run.
+//     FFT size, two per size (re-created only when a job needs more ligand atoms, another t
+//     or another batch; at most eight alive, the least recently used freed first), taking
+//     first a prepared job whose size has a live context.  One job is in flight while the
+//     next is set up: the next job's receptor, ligand and batches are queued on its own
+//     context before the GPU thread waits for the previous job's merge and writes its .out
+//     file, so the GPU runs one job's batches straight after the other's.
+// Host memory stays bounded on long tables: at most two prepared jobs per preparation
+// thread wait for the GPU, and the receptor grid cache keeps the grids of the last 16
+// receptors (and of any receptor a waiting job still uses).
 // A failed job is put back in the queue until it has run 1 + retries times
 // (src/mpidp.cpp:372-543); the log ends with the per-job table (src/mpidp.cpp:900-935).
 #include <sys/time.h>
@@ -184,17 +189,32 @@ struct Grids {
     std::vector<float> r, i;
 };
 
-// receptor grids shared by the jobs that dock the same receptor on the same grid
+// Receptor grids shared by the jobs that dock the same receptor on the same grid.  An
+// entry lives while a job holds it or while it is among the kKeep most recently used, so
+// a long table of distinct receptors does not accumulate grids.
 class GridCache {
 public:
     using Key = std::tuple<std::string, int, float, float, float, float>;
-    std::shared_ptr<const Grids> get(const Key &k, const Receptor &rec, int ng, const Parameter &p) {
+    struct Entry {
+        std::once_flag once;
+        std::shared_ptr<const Grids> grids;
+    };
+    static constexpr size_t kKeep = 16;
+    std::shared_ptr<Entry> get(const Key &k, const Receptor &rec, int ng, const Parameter &p) {
         std::shared_ptr<Entry> e;
         {
             std::lock_guard<std::mutex> g(mu_);
-            auto &slot = map_[k];
-            if (!slot) slot = std::make_shared<Entry>();
-            e = slot;
+            e = map_[k].lock();
+            if (!e) {
+                e = std::make_shared<Entry>();
+                map_[k] = e;
+            }
+            auto it = std::find(recent_.begin(), recent_.end(), e);
+            if (it != recent_.end()) recent_.erase(it);
+            recent_.push_front(e);
+            if (recent_.size() > kKeep) recent_.pop_back();
+            if (map_.size() > 4 * kKeep)  // forget the keys whose grids are gone
+                for (auto m = map_.begin(); m != map_.end();) m = m->second.expired() ? map_.erase(m) : std::next(m);
         }
         std::call_once(e->once, [&] {
             auto gr = std::make_shared<Grids>();
@@ -207,16 +227,19 @@ public:
                          p.elec_ratio, gr->r.data(), gr->i.data(), 1);  // the preparation threads run in parallel
             e->grids = gr;
         });
-        return e->grids;
+        return e;
+    }
+    size_t size() {
+        std::lock_guard<std::mutex> g(mu_);
+        size_t n = 0;
+        for (auto &m : map_) n += !m.second.expired();
+        return n;
     }
 
 private:
-    struct Entry {
-        std::once_flag once;
-        std::shared_ptr<const Grids> grids;
-    };
     std::mutex mu_;
-    std::map<Key, std::shared_ptr<Entry>> map_;
+    std::map<Key, std::weak_ptr<Entry>> map_;
+    std::deque<std::shared_ptr<Entry>> recent_;
 };
 
 struct Job {
@@ -230,7 +253,8 @@ struct Job {
     std::unique_ptr<Ligand> lig;
     int ng = 0;
     std::vector<float> gc, rotcoef;
-    std::shared_ptr<const Grids> grids;
+    std::shared_ptr<GridCache::Entry> grids;  // grids->grids: the receptor grids
+    mdk_ctx *ctx = nullptr;  // the context the job was issued on
     std::string err;
     double t_prep = 0, t_dock = 0;
     // filled by docking: the merged top-N, written to the .out file afterwards
@@ -293,64 +317,72 @@ struct Slot {
     long last_use = 0;
 };
 
-// One context per FFT size, at most kMaxLive at a time (each holds up to two batches of
-// about 8 GiB of buffers): a table with more sizes than that frees the least recently
-// used context before it creates another.
+// Two contexts per FFT size, so that a job can be set up and queued on one while the job
+// in flight still runs on the other; at most kMaxLive at a time (each holds up to two
+// batches of about 8 GiB of buffers): a table with more sizes than that frees the least
+// recently used context (never the busy one) before it creates another.
 class Contexts {
 public:
-    static constexpr int kMaxLive = 6;
+    static constexpr int kMaxLive = 8;
     ~Contexts() {
         for (auto &kv : slots_)
             if (kv.second.ctx) mdk_destroy(kv.second.ctx);
     }
-    // context for the job's FFT size, able to hold its ligand, t and batch
-    mdk_ctx *get(const Job &j, std::string &err) {
+    // a context for the job's FFT size, other than `busy`, able to hold its ligand, t and
+    // batch (its slot: the context and the rotation set loaded into it)
+    Slot *get(const Job &j, mdk_ctx *busy, std::string &err) {
         const int nf = 2 * j.ng, na = j.lig->num_atoms(), t = j.prm.num_sort;
         int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
         batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
-        Slot &s = slots_[nf];
-        s.last_use = ++clock_;
-        if (s.ctx && (s.na < na || s.t != t || s.batch < batch)) {
-            mdk_destroy(s.ctx);
-            s = Slot();
+        // of the two slots of the size: a live one that is not busy and fits, else a free one
+        Slot *s = nullptr;
+        for (int k = 0; k < 2 && !s; k++) {
+            Slot &c = slots_[{nf, k}];
+            if (c.ctx && c.ctx != busy && c.na >= na && c.t == t && c.batch >= batch) s = &c;
         }
-        if (!s.ctx) {
-            free_others(kMaxLive - 1, nf);
-            int rc = mdk_create(0, nf, j.ng, na, t, batch, &s.ctx);
-            if (rc && free_others(0, nf) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s.ctx);  // out of memory?
+        for (int k = 0; k < 2 && !s; k++) {
+            Slot &c = slots_[{nf, k}];
+            if (c.ctx != busy || !busy) s = &c;
+        }
+        s->last_use = ++clock_;
+        if (s->ctx && (s->na < na || s->t != t || s->batch < batch)) {
+            mdk_destroy(s->ctx);
+            *s = Slot();
+        }
+        if (!s->ctx) {
+            free_others(kMaxLive - 1, busy);
+            int rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);
+            if (rc && free_others(0, busy) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);  // out of memory?
             if (rc) {
                 err = std::string("mdk_create: ") + mdk_last_error();
-                s = Slot();
+                *s = Slot();
                 return nullptr;
             }
-            s.na = na;
-            s.t = t;
-            s.batch = batch;
-            s.last_use = clock_;
+            s->na = na;
+            s->t = t;
+            s->batch = batch;
+            s->last_use = clock_;
         }
-        return s.ctx;
+        return s;
     }
-    int &rot_set(int nf) { return slots_[nf].rot_set; }
     bool has(int nf) const {
-        auto it = slots_.find(nf);
-        return it != slots_.end() && it->second.ctx != nullptr;
-    }
-    int live() const {
-        int n = 0;
-        for (const auto &kv : slots_) n += kv.second.ctx != nullptr;
-        return n;
+        for (int k = 0; k < 2; k++) {
+            auto it = slots_.find({nf, k});
+            if (it != slots_.end() && it->second.ctx) return true;
+        }
+        return false;
     }
 
 private:
-    // destroy the least recently used contexts other than size `keep_nf` until at most
-    // `keep` of them are left; returns how many were destroyed
-    int free_others(int keep, int keep_nf) {
+    // destroy the least recently used contexts other than `busy` until at most `keep` of
+    // them are left; returns how many were destroyed
+    int free_others(int keep, mdk_ctx *busy) {
         int freed = 0;
         for (;;) {
             int n = 0;
             Slot *lru = nullptr;
             for (auto &kv : slots_)
-                if (kv.first != keep_nf && kv.second.ctx) {
+                if (kv.second.ctx && kv.second.ctx != busy) {
                     n++;
                     if (!lru || kv.second.last_use < lru->last_use) lru = &kv.second;
                 }
@@ -360,7 +392,7 @@ private:
             freed++;
         }
     }
-    std::map<int, Slot> slots_;
+    std::map<std::pair<int, int>, Slot> slots_;  // (nf, 0 or 1)
     long clock_ = 0;
 };
 
@@ -369,21 +401,23 @@ private:
 // -O also scores and writes the detailed output, which needs the context.  The .out file
 // is written by write_out(), from host data only, so the GPU thread can write one job's
 // file while the next job's batches run.
-int issue(Job &j, Contexts &cs) {
+int issue(Job &j, Contexts &cs, mdk_ctx *busy) {
     const double t0 = now_s();
     Parameter &p = j.prm;
-    mdk_ctx *ctx = cs.get(j, j.err);
-    if (!ctx) return 1;
+    Slot *slot = cs.get(j, busy, j.err);
+    if (!slot) return 1;
+    mdk_ctx *ctx = j.ctx = slot->ctx;
     auto abi = [&](const char *what) {
         j.err = std::string(what) + ": " + mdk_last_error();
         return 1;
     };
-    int &loaded = cs.rot_set(2 * j.ng);
+    int &loaded = slot->rot_set;
     if (loaded != p.rotation_set) {
         if (mdk_set_rotations(ctx, j.rotcoef.data(), p.num_rot_angles)) return abi("mdk_set_rotations");
         loaded = p.rotation_set;
     }
-    if (mdk_set_receptor(ctx, j.grids->r.data(), j.grids->i.data())) return abi("mdk_set_receptor");
+    const Grids &g = *j.grids->grids;
+    if (mdk_set_receptor(ctx, g.r.data(), g.i.data())) return abi("mdk_set_receptor");
     const Ligand &L = *j.lig;
     if (mdk_set_ligand_atoms(ctx, L.num_atoms(), L.coords().data(), L.center(), L.radius.data(), L.charge.data(),
                              j.gc.data(), p.grid_width, p.lig_core))
@@ -393,19 +427,19 @@ int issue(Job &j, Contexts &cs) {
     return 0;
 }
 
-int complete(Job &j, Contexts &cs) {
+// waits for the job's merged top-N on the context issue() used (still alive: Contexts
+// never frees the busy context)
+int complete(Job &j) {
     const double t0 = now_s();
     Parameter &p = j.prm;
-    mdk_ctx *ctx = cs.get(j, j.err);  // the context issue() used, still alive
-    if (!ctx) return 1;
     std::vector<mdk_sortscore> best(std::max(1, p.num_output));
-    if (mdk_topn(ctx, p.num_rot_angles, p.num_output, best.data())) {
+    if (mdk_topn(j.ctx, p.num_rot_angles, p.num_output, best.data())) {
         j.err = std::string("mdk_topn: ") + mdk_last_error();
         return 1;
     }
     j.top.resize(best.size());
     std::memcpy(j.top.data(), best.data(), sizeof(SortScore) * best.size());
-    if (p.detail_output_flag && write_detail(p, *j.rec, *j.lig, j.ng, j.top.data(), ctx, j.err)) return 1;
+    if (p.detail_output_flag && write_detail(p, *j.rec, *j.lig, j.ng, j.top.data(), j.ctx, j.err)) return 1;
     j.t_dock += now_s() - t0;
     return 0;
 }
@@ -419,10 +453,14 @@ int write_out(Job &j) {
 
 template <class T> class Queue {
 public:
+    explicit Queue(size_t cap = 0) : cap_(cap) {}
+    // blocks while the queue holds cap items (cap 0: unbounded)
     void push(T v) {
         {
-            std::lock_guard<std::mutex> g(mu_);
+            std::unique_lock<std::mutex> g(mu_);
+            space_.wait(g, [&] { return cap_ == 0 || q_.size() < cap_ || closed_; });
             q_.push_back(std::move(v));
+            high_ = std::max(high_, q_.size());
         }
         cv_.notify_one();
     }
@@ -433,6 +471,7 @@ public:
         if (q_.empty()) return false;
         v = std::move(q_.front());
         q_.pop_front();
+        space_.notify_one();
         return true;
     }
     // without waiting: false when the queue is empty, else as pop_prefer
@@ -443,6 +482,7 @@ public:
         if (it == q_.end()) it = q_.begin();
         v = std::move(*it);
         q_.erase(it);
+        space_.notify_one();
         return true;
     }
     // as pop, but takes the first item for which pred holds when there is one
@@ -454,6 +494,7 @@ public:
         if (it == q_.end()) it = q_.begin();
         v = std::move(*it);
         q_.erase(it);
+        space_.notify_one();
         return true;
     }
     void close() {
@@ -462,11 +503,18 @@ public:
             closed_ = true;
         }
         cv_.notify_all();
+        space_.notify_all();
+    }
+    // the most items the queue has held at once
+    size_t high_water() {
+        std::lock_guard<std::mutex> g(mu_);
+        return high_;
     }
 
 private:
+    size_t cap_, high_ = 0;
     std::mutex mu_;
-    std::condition_variable cv_;
+    std::condition_variable cv_, space_;
     std::deque<T> q_;
     bool closed_ = false;
 };
@@ -516,7 +564,7 @@ int run_table(int argc, char **argv, std::string &err) {
     int nprep = o.prep_threads;
     if (nprep < 1) nprep = std::max(1, std::min(8, (int)std::thread::hardware_concurrency() / 2));
 
-    Queue<std::unique_ptr<Job>> todo, ready;
+    Queue<std::unique_ptr<Job>> todo, ready(2 * (size_t)nprep);  // prepared jobs wait in a bounded queue
     std::mutex rec_mu;
     int outstanding = njobs;  // jobs neither finished nor given up (guarded by rec_mu)
     for (int i = 0; i < njobs; i++) {
@@ -579,7 +627,9 @@ int run_table(int argc, char **argv, std::string &err) {
         });
     {
         Contexts cs;
-        std::unique_ptr<Job> j, written;  // written: docked, its .out file still to write
+        // inflight: issued, its merge not yet waited for; written: merged, its .out file
+        // still to write
+        std::unique_ptr<Job> j, inflight, written;
         auto report = [&](std::unique_ptr<Job> x, bool ok) {
             std::printf("# job %05d (%s): %s\n", x->id, x->line.c_str(), ok ? "done" : x->err.c_str());
             std::fflush(stdout);
@@ -595,27 +645,39 @@ int run_table(int argc, char **argv, std::string &err) {
             }
             report(std::move(written), ok);
         };
+        auto complete_inflight = [&] {
+            if (!inflight) return;
+            bool ok = false;
+            try {
+                ok = complete(*inflight) == 0;
+            } catch (const std::exception &e) {
+                inflight->err = std::string("docking: ") + e.what();
+            }
+            write_pending();  // the job before, if any
+            if (ok) written = std::move(inflight);
+            else report(std::move(inflight), false);
+        };
         // a prepared job whose FFT size has a live context goes first: fewer contexts made
         // and freed when the table mixes more sizes than stay alive
         auto live = [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); };
         for (;;) {
             if (!ready.try_pop_prefer(j, live)) {
-                write_pending();  // nothing prepared: finish the previous job before waiting
+                complete_inflight();  // nothing prepared: finish what is queued before waiting
+                write_pending();
                 if (!ready.pop_prefer(j, live)) break;
             }
             bool ok = false;
             try {
-                ok = issue(*j, cs) == 0;
-                write_pending();  // the previous job's file, while this job's batches run
-                ok = ok && complete(*j, cs) == 0;
+                ok = issue(*j, cs, inflight ? inflight->ctx : nullptr) == 0;  // queued behind the job in flight
             } catch (const std::exception &e) {
                 j->err = std::string("docking: ") + e.what();
-                ok = false;
             }
+            complete_inflight();  // and its .out file is written while this job's batches run
             write_pending();
-            if (ok) written = std::move(j);
+            if (ok) inflight = std::move(j);
             else report(std::move(j), false);
         }
+        complete_inflight();
         write_pending();
     }
     for (auto &w : workers) w.join();
@@ -636,6 +698,7 @@ int run_table(int argc, char **argv, std::string &err) {
         else rotations += r.rotations;
     }
     const double el = now_s() - t_start;
+    log << "\nPrepared jobs waiting for the GPU: at most " << ready.high_water() << " (limit " << 2 * nprep << ")";
     log << "\nJobs done = " << njobs - failed << " / " << njobs << "\nElapsed time  = " << el << " sec.\n"
         << "Rotations docked = " << rotations << " (" << (el > 0 ? rotations / el : 0.0) << " per second)\n";
     if (failed) {

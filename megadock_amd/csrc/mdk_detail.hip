@@ -6,22 +6,35 @@
 // s = (3 ng - t) mod 2 ng per axis, ligand voxel (l, m, n) meets receptor voxel
 // (l + s - ng, ...) where that lies inside the grid, and per pose
 //   rPSC = sum Rp * Lr,  ELEC = sum Ri * Li,  ACE = sum (Rr - Rp) * Lr,
-// each split into its positive and negative products.  The products are formed in float
-// as the reference forms them; the sums run in double, one workgroup per (pose, x slab)
-// and then in slab order, so every call gives the same result.
+// each split into its positive and negative products (the decoys' lines,
+// src/docking.cpp:746-807) and each also unsplit (the lines of the poses around the input
+// position, :646-654).  As in the reference the products are formed in float and each is
+// added to its float sum in the order l, m, n ascending, so the sums are the reference's
+// bit for bit:
+// one wavefront per pose walks the rows (l, m) in order, its lanes form the products of
+// 64 consecutive n at a time, and the non-zero ones are added one after another in lane
+// order (a zero product is skipped by the reference and changes no sum).
 #include <hip/hip_runtime.h>
 
 #include "mdk_internal.h"
 
 namespace {
 
-constexpr int kDetailThreads = 256;
+// the products of `v` held by the lanes in `mask`, added to acc in lane order
+__device__ __forceinline__ float add_in_lane_order(float acc, float v, unsigned long long mask) {
+    while (mask) {  // uniform
+        const int k = __ffsll((long long)mask) - 1;
+        acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+        mask &= mask - 1;
+    }
+    return acc;
+}
 
-__global__ void __launch_bounds__(kDetailThreads)
+__global__ void __launch_bounds__(64)
     k_detail(const float2 *__restrict__ lg, const float2 *__restrict__ rec, const float *__restrict__ rpsc,
-             const int *__restrict__ trans, int ng, double *__restrict__ part) {
-    const int l = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-    const int ng3 = ng * ng * ng;
+             const int *__restrict__ trans, int ng, float *__restrict__ out) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const size_t ng3 = (size_t)ng * ng * ng;
     int s[3], lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {  // src/docking.cpp:720-744
@@ -29,53 +42,49 @@ __global__ void __launch_bounds__(kDetailThreads)
         lo[a] = s[a] < ng ? ng - s[a] : 0;
         hi[a] = s[a] < ng ? ng : 2 * ng - s[a];
     }
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // rPSC +/-, ELEC +/-, ACE +/-
-    if (l >= lo[0] && l < hi[0]) {
-        const int nm = hi[1] - lo[1], nn = hi[2] - lo[2];
-        const float2 *L = lg + (size_t)b * ng3;
-        for (int q = tid; q < nm * nn; q += kDetailThreads) {
-            const int m = lo[1] + q / nn, n = lo[2] + q % nn;
-            const int lz = (l * ng + m) * ng + n;
-            const int rz = ((l + s[0] - ng) * ng + (m + s[1] - ng)) * ng + (n + s[2] - ng);
-            const float2 lv = L[lz], rv = rec[rz];
-            const float rp = rpsc[rz];
-            const float p = rp * lv.x;               // rPSC
-            const float e = rv.y * lv.y;             // ELEC
-            const float c = (rv.x - rp) * lv.x;      // ACE
-            if (p > 0.0f) acc[0] += p; else if (p < 0.0f) acc[1] += p;
-            if (e > 0.0f) acc[2] += e; else if (e < 0.0f) acc[3] += e;
-            if (c > 0.0f) acc[4] += c; else if (c < 0.0f) acc[5] += c;
+    float acc[9] = {};  // rPSC +/-, ELEC +/-, ACE +/-, then rPSC, ELEC, ACE unsplit
+    const float2 *L = lg + (size_t)b * ng3;
+    for (int l = lo[0]; l < hi[0]; l++)
+        for (int m = lo[1]; m < hi[1]; m++) {
+            const size_t lrow = ((size_t)l * ng + m) * ng;
+            const size_t rrow = ((size_t)(l + s[0] - ng) * ng + (m + s[1] - ng)) * ng + (s[2] - ng);
+            for (int n0 = lo[2]; n0 < hi[2]; n0 += 64) {
+                const int n = n0 + lane;
+                float p = 0.0f, e = 0.0f, c = 0.0f;
+                if (n < hi[2]) {
+                    const float2 lv = L[lrow + n], rv = rec[rrow + n];
+                    const float rp = rpsc[rrow + n];
+                    p = rp * lv.x;           // rPSC
+                    e = rv.y * lv.y;         // ELEC
+                    c = (rv.x - rp) * lv.x;  // ACE
+                }
+                const unsigned long long any = __ballot(p != 0.0f || e != 0.0f || c != 0.0f);
+                if (!any) continue;  // uniform: nothing to add in this stretch of the row
+                acc[0] = add_in_lane_order(acc[0], p, __ballot(p > 0.0f));
+                acc[1] = add_in_lane_order(acc[1], p, __ballot(p < 0.0f));
+                acc[2] = add_in_lane_order(acc[2], e, __ballot(e > 0.0f));
+                acc[3] = add_in_lane_order(acc[3], e, __ballot(e < 0.0f));
+                acc[4] = add_in_lane_order(acc[4], c, __ballot(c > 0.0f));
+                acc[5] = add_in_lane_order(acc[5], c, __ballot(c < 0.0f));
+                acc[6] = add_in_lane_order(acc[6], p, __ballot(p != 0.0f));
+                acc[7] = add_in_lane_order(acc[7], e, __ballot(e != 0.0f));
+                acc[8] = add_in_lane_order(acc[8], c, __ballot(c != 0.0f));
+            }
         }
-    }
-    __shared__ double red[6][kDetailThreads];
+    if (lane < 9) {
+        float v = acc[0];
 #pragma unroll
-    for (int k = 0; k < 6; k++) red[k][tid] = acc[k];
-    __syncthreads();
-    for (int h = kDetailThreads / 2; h >= 1; h >>= 1) {  // fixed tree: deterministic
-        if (tid < h)
-#pragma unroll
-            for (int k = 0; k < 6; k++) red[k][tid] += red[k][tid + h];
-        __syncthreads();
+        for (int k = 1; k < 9; k++)
+            if (lane == k) v = acc[k];
+        out[(size_t)b * 9 + lane] = v;
     }
-    if (tid < 6) part[((size_t)b * ng + l) * 6 + tid] = red[tid][0];
-}
-
-__global__ void k_detail_finish(const double *__restrict__ part, int ng, double *__restrict__ out) {
-    const int b = blockIdx.x, k = threadIdx.x;
-    if (k >= 6) return;
-    double v = 0.0;
-    for (int l = 0; l < ng; l++) v += part[((size_t)b * ng + l) * 6 + k];
-    out[(size_t)b * 6 + k] = v;
 }
 
 }  // namespace
 
-size_t detail_scratch_doubles(int max_batch, int ng) { return (size_t)max_batch * ng * 6 + (size_t)max_batch * 6; }
-
 hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
-                         double *scratch, double *out_dev, hipStream_t s) {
-    k_detail<<<dim3(ng, B), kDetailThreads, 0, s>>>(lg, recgrid, rpsc, trans_dev, ng, scratch);
-    k_detail_finish<<<B, 64, 0, s>>>(scratch, ng, out_dev);
+                         float *out_dev, hipStream_t s) {
+    k_detail<<<B, 64, 0, s>>>(lg, recgrid, rpsc, trans_dev, ng, out_dev);
     return hipGetLastError();
 }
 

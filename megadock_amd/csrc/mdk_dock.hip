@@ -29,9 +29,8 @@
 #include "../../include/mdk_hip.h"
 
 // real-space detail sums (mdk_detail.hip)
-size_t detail_scratch_doubles(int max_batch, int ng);
 hipError_t launch_detail(const float2 *lg, const float2 *recgrid, const float *rpsc, const int *trans_dev, int B, int ng,
-                         double *scratch, double *out_dev, hipStream_t s);
+                         float *out_dev, hipStream_t s);
 
 hipError_t launch_decoy_coords(const float *xyz, int na, const float *pre, const float *rot, const int *trans, int nd,
                                float spacing, const float *r, float *out, hipStream_t s);
@@ -453,7 +452,7 @@ struct mdk_ctx {
     int xcorr_variant = 6;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
     int cu_count = 256;       // compute units (persistent grids)
     int inv_variant = 2;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
-    int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, 1 separate passes
+    int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, 1 separate passes, 2 fused, prefetching
     float2 *d_rec = nullptr;
     bool have_rec = false, have_lig = false;
     // ligand
@@ -480,12 +479,13 @@ struct mdk_ctx {
     double *d_gpart = nullptr;    // partial sums
     float *d_rpsc = nullptr;      // receptor shape grid without the ACE term (mdk_score_detail), ng^3
     int *d_btrans = nullptr;      // grid translations of a batch (max_batch * 3)
-    double *d_dpart = nullptr;    // detail sums: per (pose, slab) partials, then per pose
+    float *d_dsum = nullptr;      // detail sums of a batch (max_batch * 9)
     bool have_recgrid = false;
     void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
     mdk_sortscore *d_topn_out = nullptr;
     int topn_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> tev;  // mdk_dock_range_timed: MDK_NSTAGE + 1 events per batch
 };
 
 namespace {
@@ -503,11 +503,13 @@ VoxGrid vox_grid(const mdk_ctx *c, const BatchSlot &S) {
 // launch_voxelize has voxelized; output top-t candidates (or the score map).
 // ev (optional): events recorded after each of the five FFT-side stages
 // spectrum_only: stop after the fold, leaving Q[b][kx][ky][kz] in S.d_cc (specialised sizes)
+// from_lg: the ligand grids are already in S.d_lg (given by the host, mdk_dock_grid); the
+// generic kernels read them there
 int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_t *ev = nullptr,
-                     bool spectrum_only = false) {
+                     bool spectrum_only = false, bool from_lg = false) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     hipStream_t s = S.stream;
-    if (c->use_pipe) {
+    if (c->use_pipe && !from_lg) {
         PipeArgs a;
         a.B = B;
         a.ng = ng;
@@ -534,7 +536,7 @@ int run_fft_pipeline(mdk_ctx *c, BatchSlot &S, int B, float *smap_dev, hipEvent_
         HIPCHK(pipe_run(nf, a));
         return 0;
     }
-    HIPCHK(launch_vox_expand(vox_grid(c, S), B, S.d_lg, s));
+    if (!from_lg) HIPCHK(launch_vox_expand(vox_grid(c, S), B, S.d_lg, s));
     const int lines_z = B * ng * ng;
     k_fwd_z<<<(lines_z + WZ - 1) / WZ, NT, lds_bytes(c, WZ), s>>>(S.d_lg, S.d_a1, c->plan, c->d_tw, ng, nd, lines_z);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
@@ -625,6 +627,11 @@ template <typename T> int dmalloc(T **p, size_t n) {
 
 }  // namespace
 
+// the copy of mdk_copy_probe: n4 16-byte words, grid-stride
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) b[i] = a[i];
+}
+
 // ==================================================================== C ABI
 extern "C" {
 
@@ -713,7 +720,7 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
         dmalloc(&c->d_brot, 9 * B) || dmalloc(&c->d_bid, B) || dmalloc(&c->d_recgrid, ng3) ||
         dmalloc(&c->d_wshape, na) || dmalloc(&c->d_bt, 3 * B) || dmalloc(&c->d_gout, 7 * B) ||
         dmalloc(&c->d_gpart, grad_scratch_doubles((int)B)) || dmalloc(&c->d_rpsc, ng3) || dmalloc(&c->d_btrans, 3 * B) ||
-        dmalloc(&c->d_dpart, detail_scratch_doubles((int)B, ng)))
+        dmalloc(&c->d_dsum, 9 * B))
         return bail(1);
     for (int k = 0; k < c->nslots; k++) {
         BatchSlot &S = c->slot[k];
@@ -784,7 +791,7 @@ int mdk_destroy(mdk_ctx *c) {
         if (c->slot[k].stream) (void)hipStreamSynchronize(c->slot[k].stream);
     void *ptrs[] = {c->d_tw, c->d_tw4, c->d_rec, c->d_xyz, c->d_rc2, c->d_rs2, c->d_charge, c->d_gc, c->d_rot,
                     c->d_rotid, c->d_brot, c->d_bid, c->d_results, c->d_topn_scratch, c->d_topn_out,
-                    c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart, c->d_rpsc, c->d_btrans, c->d_dpart};
+                    c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart, c->d_rpsc, c->d_btrans, c->d_dsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     drop_graphs(c);
@@ -799,6 +806,7 @@ int mdk_destroy(mdk_ctx *c) {
     }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     delete c;
     return 0;
 }
@@ -975,6 +983,31 @@ int mdk_dock(mdk_ctx *c, const float *rotcoef, const int *rot_id, int B, mdk_sor
     return 0;
 }
 
+int mdk_dock_grid(mdk_ctx *c, const float *grid_r, const float *grid_i, int rot_id, mdk_sortscore *out) {
+    if (!c || !grid_r || !grid_i || !out) return fail("mdk_dock_grid: null argument");
+    if (!c->have_rec) return fail("mdk_dock_grid: receptor must be set first");
+    HIPCHK(hipSetDevice(c->device));
+    BatchSlot &S = c->slot[0];
+    const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
+    std::vector<float2> h(ng3);
+    for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
+    HIPCHK(hipMemcpyAsync(S.d_lg, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_bid, &rot_id, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int nf = c->nf;
+    if (c->t > MDK_MAXT) {  // the score map, then the per-rotation select
+        if (run_fft_pipeline(c, S, 1, S.d_smap, nullptr, false, true)) return 1;
+        HIPCHK(launch_topt(S.d_smap, 1, nf, c->t, S.d_tsel, S.d_tkeys, S.d_bout, c->d_bid, nullptr, c->stream));
+    } else {
+        if (run_fft_pipeline(c, S, 1, nullptr, nullptr, false, true)) return 1;
+        k_reduce_top<<<1, NT, 0, c->stream>>>(S.d_cand, S.d_bout, c->d_bid, nullptr, nf * nf, c->t, nf,
+                                             (float)((size_t)nf * nf * nf));
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(out, S.d_bout, sizeof(mdk_sortscore) * c->t, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));  // h is read by the upload until here
+    return 0;
+}
+
 int mdk_set_rotations(mdk_ctx *c, const float *rotcoef, int R) {
     if (!c || !rotcoef || R < 1) return fail("mdk_set_rotations: invalid argument");
     HIPCHK(hipSetDevice(c->device));
@@ -1099,6 +1132,50 @@ int mdk_time_range(mdk_ctx *c, int first, int count, int reps, float *ms) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = t / reps;
+    return 0;
+}
+
+int mdk_dock_range_timed(mdk_ctx *c, int first, int count, float *stage_ms, int *nstages, int *nfull) {
+    if (!c || !stage_ms || !nstages || !nfull) return fail("mdk_dock_range_timed: null argument");
+    if (!c->have_rec || !c->have_lig || !c->d_rot) return fail("mdk_dock_range_timed: receptor, ligand and rotations must be set first");
+    if (first < 0 || count < 0 || first + count > c->nrot) return fail("mdk_dock_range_timed: range outside the rotation table");
+    HIPCHK(hipSetDevice(c->device));
+    const int nb = (count + c->max_batch - 1) / c->max_batch;
+    const size_t need = (size_t)nb * (MDK_NSTAGE + 1);
+    while (c->tev.size() < need) {  // the pool grows once to the largest range timed
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreate(&e));
+        c->tev.push_back(e);
+    }
+    // as mdk_dock_range, always with eager launches, events around every stage of every batch
+    HIPCHK(hipEventRecord(c->slot[0].done, c->stream));
+    for (int k = 1; k < c->nslots; k++) HIPCHK(hipStreamWaitEvent(c->slot[k].stream, c->slot[0].done, 0));
+    int q = 0;
+    for (int b0 = first; b0 < first + count; b0 += c->max_batch, q++) {
+        const int B = std::min(c->max_batch, first + count - b0);
+        if (dock_batch(c, c->slot[q % c->nslots], c->d_rot + 9 * (size_t)b0, c->d_rotid + b0, B,
+                       c->d_results + (size_t)b0 * c->t, &c->tev[(size_t)q * (MDK_NSTAGE + 1)]))
+            return 1;
+    }
+    for (int k = 1; k < c->nslots; k++) {
+        HIPCHK(hipEventRecord(c->slot[k].done, c->slot[k].stream));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->slot[k].done, 0));
+    }
+    for (int k = 0; k < c->nslots; k++) HIPCHK(hipStreamSynchronize(c->slot[k].stream));
+    for (int k = 0; k < MDK_NSTAGE; k++) stage_ms[k] = 0.0f;
+    int full = 0;
+    for (int b = 0; b < q; b++) {
+        if (b * c->max_batch + c->max_batch > count) break;  // the last, partial batch is left out
+        const hipEvent_t *ev = &c->tev[(size_t)b * (MDK_NSTAGE + 1)];
+        for (int k = 0; k < MDK_NSTAGE; k++) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            stage_ms[k] += ms;
+        }
+        full++;
+    }
+    *nstages = MDK_NSTAGE;
+    *nfull = full;
     return 0;
 }
 
@@ -1237,7 +1314,7 @@ int mdk_score_companion(mdk_ctx *c, const float *rotcoef, const float *t, int B,
     return 0;
 }
 
-int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, double *out) {
+int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, const int *trans, int B, float *out) {
     if (!c || !rec_rpsc || !rotcoef || !trans || !out) return fail("mdk_score_detail: null argument");
     if (!c->have_recgrid || !c->have_lig)
         return fail("mdk_score_detail: receptor grids (mdk_set_receptor) and ligand must be set first");
@@ -1249,7 +1326,6 @@ int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, co
     const size_t ng3 = (size_t)ng * ng * ng;
     BatchSlot &S = c->slot[0];
     HIPCHK(hipMemcpyAsync(c->d_rpsc, rec_rpsc, sizeof(float) * ng3, hipMemcpyHostToDevice, c->stream));
-    double *sums = c->d_dpart + (size_t)c->max_batch * ng * 6;
     for (int q0 = 0; q0 < B; q0 += c->max_batch) {  // batches of the context's size
         const int nb = B - q0 < c->max_batch ? B - q0 : c->max_batch;
         HIPCHK(hipMemcpyAsync(c->d_brot, rotcoef + 9 * (size_t)q0, sizeof(float) * 9 * nb, hipMemcpyHostToDevice,
@@ -1258,8 +1334,8 @@ int mdk_score_detail(mdk_ctx *c, const float *rec_rpsc, const float *rotcoef, co
                               c->stream));
         HIPCHK(launch_voxelize(c->lig, c->d_brot, nb, S.vs, S.d_vidx, S.d_chg, c->stream));
         HIPCHK(launch_vox_expand(vox_grid(c, S), nb, S.d_lg, c->stream));
-        HIPCHK(launch_detail(S.d_lg, c->d_recgrid, c->d_rpsc, c->d_btrans, nb, ng, c->d_dpart, sums, c->stream));
-        HIPCHK(hipMemcpyAsync(out + 6 * (size_t)q0, sums, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(launch_detail(S.d_lg, c->d_recgrid, c->d_rpsc, c->d_btrans, nb, ng, c->d_dsum, c->stream));
+        HIPCHK(hipMemcpyAsync(out + 9 * (size_t)q0, c->d_dsum, sizeof(float) * 9 * nb, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));  // the staging buffers are reused by the next batch
     }
     return 0;
@@ -1298,6 +1374,7 @@ int mdk_decoy_coords(int device, const float *xyz, int na, const float *rot_pre,
 
 int mdk_copy_probe(int device, size_t bytes, int reps, float *ms) {
     if (!ms || bytes == 0 || reps < 1) return fail("mdk_copy_probe: invalid argument");
+    if (bytes % 16) return fail("mdk_copy_probe: bytes must be a multiple of 16");
     HIPCHK(hipSetDevice(device));
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1306,10 +1383,17 @@ int mdk_copy_probe(int device, size_t bytes, int reps, float *ms) {
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = fail("mdk_copy_probe: allocation failed");
     if (!rc && hipMemset(a, 1, bytes) != hipSuccess) rc = fail("mdk_copy_probe: memset failed");
-    if (!rc && hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice) != hipSuccess) rc = fail("mdk_copy_probe: copy failed");
+    // one dispatch per copy, so that a profiler's counters of one dispatch see all its bytes
+    const size_t n4 = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, 16384);
+    if (!rc) {
+        k_copy_probe<<<blocks, 256>>>(static_cast<const float4 *>(a), static_cast<float4 *>(b), n4);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = fail("mdk_copy_probe: copy failed");
+    }
     if (!rc) {
         (void)hipEventRecord(e0, nullptr);
-        for (int r = 0; r < reps; r++) (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, nullptr);
+        for (int r = 0; r < reps; r++)
+            k_copy_probe<<<blocks, 256>>>(static_cast<const float4 *>(a), static_cast<float4 *>(b), n4);
         (void)hipEventRecord(e1, nullptr);
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
             rc = fail("mdk_copy_probe: timing failed");

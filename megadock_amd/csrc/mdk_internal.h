@@ -283,7 +283,7 @@ struct PipeArgs {
     int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches, 2 and 3 as 0 and 1
                         // with running top-t lists over contiguous planes (MDK_INV_VARIANT)
     bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
-    int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, 1 separate (MDK_FWD_VARIANT)
+    int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, 1 separate, 2 fused with the next plane prefetched (MDK_FWD_VARIANT)
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.

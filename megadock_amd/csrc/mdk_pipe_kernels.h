@@ -117,7 +117,9 @@ template <int N, int C> constexpr bool fwd_zy_fits() {
            (size_t)N * (NG + 1) * 8 + (size_t)N * (C + 1) * 8 + (size_t)N * 8 <= 150 * 1024;
 }
 
-template <int N, int C>
+// PF: the next plane's mask words and charges are loaded while the current plane is
+// transformed (their zeroing of the charge grid follows when they are used).
+template <int N, int C, bool PF = false>
 __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict__ a2, const float2 *__restrict__ twg,
                                                    int nplanes) {
     constexpr int NG = N / 2, ND = N / 4;
@@ -126,32 +128,55 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
     static_assert(fwd_zy_fits<N, C>(), "fused forward pass does not fit");
     // one spare row after each group of N2 in the z exchange, where it still fits
     constexpr int PZ = (size_t)(N + N1) * LA * 8 + (size_t)N * LE * 8 + (size_t)N * 8 <= 150 * 1024 ? 1 : 0;
+    constexpr int RW = (NG + 31) / 32;
+    static_assert(RW <= 4, "row of at most 128 voxels");
     __shared__ float2 A[(N + N1 * PZ) * LA];  // z-transformed plane, [kz][y]
     __shared__ float2 E[N * LE];  // y-exchange of one chunk of columns
     __shared__ float2 tw[N];
     for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
+    // the z-row task of this thread: row y of a plane, step-1 index u (u fastest, so that a
+    // row's voxels sit in neighbouring lanes)
+    const int zy = (int)threadIdx.x / T, zu = (int)threadIdx.x - zy * T;
+    const bool zact = zy < NG && zu < N2;
+    // a row's raw voxel data: its mask words and the charges of this thread's voxels
+    uint32_t rd[RW], ro[RW];
+    float rq[N1];
+    auto load_row = [&](int p) {
+#pragma unroll
+        for (int w = 0; w < RW; w++) rd[w] = ro[w] = 0u;
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) rq[j1] = 0.0f;
+        if (zact && p < nplanes) {
+            const int b = p / NG, x = p - b * NG, row = x * NG + zy;
+            const uint32_t *dr = vg.dmo + (size_t)b * 2 * vg.mw + (size_t)row * RW;
+#pragma unroll
+            for (int w = 0; w < RW; w++) {
+                rd[w] = dr[w];
+                ro[w] = dr[vg.mw + w];
+            }
+            const float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)row * NG;
+#pragma unroll
+            for (int j1 = 0; j1 < N1; j1++) {
+                const int e = zu + N2 * j1;
+                if (e >= ND && e < ND + NG) rq[j1] = cr[e - ND];
+            }
+        }
+    };
+    if (PF) load_row(blockIdx.x);
     for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
         const int b = p / NG, x = p - b * NG;
         // opaque thread index: keeps the per-thread addresses out of the plane loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
+        if (!PF) load_row(p);
         __syncthreads();  // A, E (and tw) free
-        // ---- z-FFT of row y: thread (y, u), u fastest so that a row's voxels sit in neighbouring lanes
+        // ---- z-FFT of row y: thread (y, u), the row's voxel values from the mask bits and
+        // the charges into the step-1 registers; each non-zero charge read is cleared
         {
             const int y = tid / T, u = tid - y * T;
             const bool zrow = y < NG;
-            const int row = x * NG + y;
             if (zrow && u < N2) {
-                // the row's mask words once, then one bit per voxel; the charges one by one
-                constexpr int RW = (NG + 31) / 32;
-                static_assert(RW <= 4, "row of at most 128 voxels");
-                const uint32_t *dr = vg.dmo + (size_t)b * 2 * vg.mw + (size_t)row * RW;
-                const int mw = vg.mw;
-                const uint32_t d0 = dr[0], o0 = dr[mw];
-                const uint32_t d1 = RW > 1 ? dr[1] : 0u, o1 = RW > 1 ? dr[mw + 1] : 0u;
-                const uint32_t d2 = RW > 2 ? dr[2] : 0u, o2 = RW > 2 ? dr[mw + 2] : 0u;
-                const uint32_t d3 = RW > 3 ? dr[3] : 0u, o3 = RW > 3 ? dr[mw + 3] : 0u;
-                float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)row * NG;
+                float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)(x * NG + y) * NG;
                 const float delta = vg.delta;
                 float2 v[N1] = {};
 #pragma unroll
@@ -159,14 +184,15 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
                     const int e = u + N2 * j1;
                     if (e >= ND && e < ND + NG) {
                         const int k = e - ND, w = k >> 5;
-                        const uint32_t d = w == 0 ? d0 : w == 1 ? d1 : w == 2 ? d2 : d3;
-                        const uint32_t o = w == 0 ? o0 : w == 1 ? o1 : w == 2 ? o2 : o3;
+                        const uint32_t d = w == 0 ? rd[0] : w == 1 ? rd[RW > 1 ? 1 : 0] : w == 2 ? rd[RW > 2 ? 2 : 0] : rd[RW > 3 ? 3 : 0];
+                        const uint32_t o = w == 0 ? ro[0] : w == 1 ? ro[RW > 1 ? 1 : 0] : w == 2 ? ro[RW > 2 ? 2 : 0] : ro[RW > 3 ? 3 : 0];
                         const uint32_t bit = 1u << (k & 31);
-                        const float q = cr[k];
+                        const float q = rq[j1];
                         if (__float_as_uint(q) != 0u) cr[k] = 0.0f;  // keep the charge grid zero
                         v[j1] = make_float2((d & bit) ? delta : ((o & bit) ? 1.0f : 0.0f), q);
                     }
                 }
+                if (PF) load_row(p + gridDim.x);  // in flight while this plane is transformed
                 rf::step1_g<N, false, PZ>(v, A, tw, 1, LA, 1, y, u);
             }
             __syncthreads();
@@ -805,10 +831,13 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     constexpr int CZY = fwd_zy_chunk<N>();
     bool fused_fwd = false;
     if constexpr (CZY > 0) {
-        if (!FULL || a.fwd_variant == 0) {  // fused z + y
+        if (!FULL || a.fwd_variant == 0 || a.fwd_variant == 2) {  // fused z + y (2: prefetching)
             const int nplanes = B * ng;
             const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;
-            kp_fwd_zy<N, CZY><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+            if (FULL && a.fwd_variant == 2)
+                kp_fwd_zy<N, CZY, true><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+            else
+                kp_fwd_zy<N, CZY><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
             if (a.ev) (void)hipEventRecord(a.ev[0], s);  // the z stage is fused into the y stage
             fused_fwd = true;
         }

@@ -1,21 +1,29 @@
-"""Summaries of rocprofv3 runs of the docking pipeline (kernel times and HBM traffic).
+"""Summaries of rocprofv3 runs of the docking pipeline (kernel times, HBM traffic and SQ
+counters).
 
-    python -m megadock_amd.profiling trace  DIR            # --kernel-trace --stats, -f csv
-    python -m megadock_amd.profiling traffic FETCH_DIR WRITE_DIR [--copy-fetch-factor 2.0]
-                                             [--copy-write-factor 1.0] --batch B --json OUT
+    python -m megadock_amd.profiling trace DIR           # --kernel-trace --stats, -f csv
+    python -m megadock_amd.profiling pmc --batch B --json OUT DIR [DIR ...]
 
-DIR, FETCH_DIR and WRITE_DIR are rocprofv3 output folders written with `-f csv -o run`.
-The traffic summary keeps, for each kernel, the dispatches of the largest grid (full
-batches), averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over them and scales each
-by the factor a plain device-to-device copy of known size needs to read its true bytes.
-bench.py reads the JSON to report the measured HBM bytes next to the algorithmic ones.
+DIR are rocprofv3 output folders written with `-f csv -o run`.  `pmc` reads the counter
+passes of one command (FETCH_SIZE in one pass, WRITE_SIZE in another, SQ counters in
+others), keeps for each kernel the dispatches of its largest grid (full batches of the
+pipeline), averages each counter over them and writes, per kernel and per dispatch:
+    hbm_bytes = FETCH_SIZE * 1024 * f_fetch + WRITE_SIZE * 1024 * f_write
+where f_fetch and f_write are the factors a plain device-to-device copy of known size in
+the same run (bench.py --copy-probe: one dispatch of `k_copy_probe` reading and writing
+1 GiB) needs for its counters to read its true bytes; and every SQ counter as read.  bench.py reads the JSON
+to report the measured HBM bytes and VALU instructions of its dominant kernel.
 """
 import argparse
 import collections
 import csv
 import json
 import os
+import re
 import sys
+
+COPY_KERNEL = "k_copy_probe"
+PIPELINE = re.compile(r"kp_|k_vox|k_reduce|k_fwd|k_x_corr|k_inv|k_topt")
 
 
 def short_name(kernel):
@@ -50,52 +58,52 @@ def dispatch_stats(trace_dir):
     return rows
 
 
-def counter_per_kernel(pmc_dir, counter):
-    vals = collections.defaultdict(list)
+def counter_values(pmc_dir):
+    """{counter: {kernel: [(grid, value per dispatch), ...]}} of one counter pass."""
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
     for r in csv.DictReader(open(os.path.join(pmc_dir, "run_counter_collection.csv"))):
-        if r["Counter_Name"] != counter:
-            continue
-        vals[short_name(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
-    out = {}
-    for k, v in vals.items():
-        g = max(x[0] for x in v)
-        sel = [x[1] for x in v if x[0] == g]
-        out[k] = {"grid": g, "kib": sum(sel) / len(sel), "dispatches": len(sel)}
-    return out
+        vals[r["Counter_Name"]][short_name(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return vals
 
 
-def traffic(fetch_dir, write_dir, batch, ff, wf, min_dispatches=4):
-    """Per-batch HBM bytes of the kernels dispatched at least min_dispatches times at their
-    largest grid (the per-batch pipeline; set-up kernels such as the receptor transform
-    run once per context)."""
-    fetch = counter_per_kernel(fetch_dir, "FETCH_SIZE")
-    write = counter_per_kernel(write_dir, "WRITE_SIZE")
-    kernels = {}
-    for k in sorted(set(fetch) | set(write)):
-        n = max(fetch.get(k, {}).get("dispatches", 0), write.get(k, {}).get("dispatches", 0))
-        if k.startswith("__amd_rocclr") or n < min_dispatches:
-            continue
-        fb = fetch.get(k, {}).get("kib", 0.0) * 1024 * ff
-        wb = write.get(k, {}).get("kib", 0.0) * 1024 * wf
-        kernels[k] = {"fetch_bytes": round(fb), "write_bytes": round(wb),
-                      "dispatches": fetch.get(k, write.get(k, {})).get("dispatches", 0)}
-    total = sum(v["fetch_bytes"] + v["write_bytes"] for v in kernels.values())
-    return {"batch": batch, "copy_calibration": {"FETCH_SIZE": ff, "WRITE_SIZE": wf},
-            "bytes_per_batch": total, "bytes_per_rotation": total / batch, "kernels": kernels}
+def largest_grid_mean(v):
+    g = max(x[0] for x in v)
+    sel = [x[1] for x in v if x[0] == g]
+    return g, sum(sel) / len(sel), len(sel)
 
 
-def counters(dirs, batch, kernels_like="kp_|k_vox|k_reduce|k_clear"):
-    """Counter values per full-batch dispatch for every counter found in the given pmc
-    output folders, divided by the batch (per rotation)."""
-    import re
-    table = collections.defaultdict(dict)
+def pmc_summary(dirs, batch, copy_bytes=1 << 30, min_dispatches=4):
+    counters = collections.defaultdict(dict)  # kernel -> counter -> (grid, mean, n)
     for d in dirs:
-        names = set(r["Counter_Name"] for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
-        for c in names:
-            for k, v in counter_per_kernel(d, c).items():
-                if re.match(kernels_like, k) and v["dispatches"] >= 4:
-                    table[k][c] = v["kib"] / batch  # the value per dispatch (named kib by the helper)
-    return table
+        for c, per_kernel in counter_values(d).items():
+            for k, v in per_kernel.items():
+                counters[k][c] = largest_grid_mean(v)
+    factors = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        got = counters.get(COPY_KERNEL, {}).get(c)
+        factors[c] = (copy_bytes / 1024.0) / got[1] if got and got[1] > 0 else None
+    # of the template variants of one kernel (autotuning runs them all), keep those
+    # dispatched as often as the most dispatched one: the variants the timed passes ran
+    most = collections.defaultdict(int)
+    for k, cs in counters.items():
+        most[k.split("<")[0]] = max(most[k.split("<")[0]], max(n for _, _, n in cs.values()))
+    kernels = {}
+    for k, cs in counters.items():
+        n = max(n for _, _, n in cs.values())
+        if not PIPELINE.match(k) or n < min_dispatches or n < most[k.split("<")[0]]:
+            continue
+        row = {"grid": max(g for g, _, _ in cs.values()), "dispatches": max(n for _, _, n in cs.values())}
+        for c, (_, mean, _) in cs.items():
+            row[c] = mean
+        if "FETCH_SIZE" in cs and "WRITE_SIZE" in cs and all(factors.values()):
+            row["fetch_bytes"] = cs["FETCH_SIZE"][1] * 1024 * factors["FETCH_SIZE"]
+            row["write_bytes"] = cs["WRITE_SIZE"][1] * 1024 * factors["WRITE_SIZE"]
+            row["hbm_bytes"] = row["fetch_bytes"] + row["write_bytes"]
+        kernels[k] = row
+    out = {"batch": batch, "copy_calibration": {"bytes": copy_bytes, "factors": factors}, "kernels": kernels}
+    if kernels and all("hbm_bytes" in v for v in kernels.values()):
+        out["pipeline_hbm_bytes"] = sum(v["hbm_bytes"] for v in kernels.values())
+    return out
 
 
 def main(argv=None):
@@ -103,25 +111,12 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("trace")
     t.add_argument("dir")
-    f = sub.add_parser("traffic")
-    f.add_argument("fetch_dir")
-    f.add_argument("write_dir")
-    f.add_argument("--batch", type=int, required=True)
-    f.add_argument("--copy-fetch-factor", type=float, default=2.0)
-    f.add_argument("--copy-write-factor", type=float, default=1.0)
-    f.add_argument("--json", required=True)
-    c = sub.add_parser("counters")
-    c.add_argument("dirs", nargs="+")
-    c.add_argument("--batch", type=int, required=True)
+    p = sub.add_parser("pmc")
+    p.add_argument("dirs", nargs="+")
+    p.add_argument("--batch", type=int, required=True)
+    p.add_argument("--copy-bytes", type=int, default=1 << 30)
+    p.add_argument("--json", required=True)
     a = ap.parse_args(argv)
-    if a.cmd == "counters":
-        tab = counters(a.dirs, a.batch)
-        cols = sorted(set(x for v in tab.values() for x in v))
-        print("Per rotation (counter value of a full-batch dispatch / batch):\n")
-        print("| kernel | " + " | ".join(cols) + " |\n|---|" + "---:|" * len(cols))
-        for k in sorted(tab, key=lambda k: -tab[k].get("SQ_INSTS_VALU", 0)):
-            print("| %s | " % k + " | ".join("%.4g" % tab[k].get(x, float("nan")) for x in cols) + " |")
-        return 0
     if a.cmd == "trace":
         print("rocprofv3 --stats summary (all dispatches):\n")
         print("| kernel | calls | avg (us) | total (ms) | % |\n|---|---:|---:|---:|---:|")
@@ -134,15 +129,23 @@ def main(argv=None):
                 continue
             print("| %s | %d | %d | %.1f | %.1f | %.2f |" % (r["kernel"], r["grid"], r["calls"], r["avg_us"],
                                                              r["median_us"], r["total_ms"]))
-    else:
-        s = traffic(a.fetch_dir, a.write_dir, a.batch, a.copy_fetch_factor, a.copy_write_factor)
-        with open(a.json, "w") as fh:
-            json.dump(s, fh, indent=1)
-        print("| kernel | fetch (MB/batch) | write (MB/batch) |\n|---|---:|---:|")
-        for k, v in s["kernels"].items():
-            print("| %s | %.1f | %.1f |" % (k, v["fetch_bytes"] / 1e6, v["write_bytes"] / 1e6))
-        print("| total | %.1f MB per batch of %d | %.2f MB per rotation |"
-              % (s["bytes_per_batch"] / 1e6, a.batch, s["bytes_per_rotation"] / 1e6))
+        return 0
+    s = pmc_summary(a.dirs, a.batch, a.copy_bytes)
+    with open(a.json, "w") as fh:
+        json.dump(s, fh, indent=1)
+    f = s["copy_calibration"]["factors"]
+    print("Copy calibration (%d bytes copied): FETCH_SIZE x %s, WRITE_SIZE x %s\n"
+          % (s["copy_calibration"]["bytes"], f["FETCH_SIZE"], f["WRITE_SIZE"]))
+    cols = sorted({c for v in s["kernels"].values() for c in v if c.isupper() or c.startswith("SQ_")})
+    print("Per full-batch dispatch (%d rotations), mean over the dispatches of the largest grid:\n" % a.batch)
+    print("| kernel | dispatches | HBM MB | " + " | ".join(cols) + " |\n|---|---:|---:|" + "---:|" * len(cols))
+    for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1].get("hbm_bytes", 0)):
+        hb = "%.1f" % (v["hbm_bytes"] / 1e6) if "hbm_bytes" in v else "-"
+        print("| %s | %d | %s | " % (k, v["dispatches"], hb) + " | ".join("%.4g" % v[c] if c in v else "-" for c in cols) + " |")
+    if "pipeline_hbm_bytes" in s:
+        print("\nPipeline: %.1f MB per batch, %.2f MB per rotation"
+              % (s["pipeline_hbm_bytes"] / 1e6, s["pipeline_hbm_bytes"] / a.batch / 1e6))
+    return 0
 
 
 if __name__ == "__main__":

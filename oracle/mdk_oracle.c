@@ -677,3 +677,56 @@ int or_dock(int ng, int nf, float gw, int na, const float *coord, const float *c
     free(gc);
     return 0;
 }
+
+/* A15 (-O): the real-space sums of Docking::output_detail for one pose, as the reference
+ * forms them (src/docking.cpp:720-801): per axis s = (3 ng - t) mod 2 ng and the ligand
+ * range [lo, hi) of :725-744; for l, m, n ascending the float products
+ *   rpsc = rec_rpsc * lig_r,  elec = rec_i * lig_i,  ace = (rec_r - rec_rpsc) * lig_r
+ * and each added, when negative or positive, to its float sum in that order (the decoys'
+ * sums, :746-807), and each added to an unsplit float sum (the sums of the poses around the
+ * input position, :646-654).  out = (rPSC+, rPSC-, ELEC+, ELEC-, ACE+, ACE-, rPSC, ELEC,
+ * ACE); cnt (optional) the six counts of the split sums. */
+void or_detail_sums_f32(int ng, const float *rec_r, const float *rec_i, const float *rec_rpsc, const float *lig_r,
+                        const float *lig_i, const int trans[3], float out[9], int cnt[6])
+{
+    const int ng2 = ng * ng;
+    int s[3], lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        s[a] = (3 * ng - trans[a]) % (2 * ng);
+        if (s[a] < ng) {
+            lo[a] = ng - s[a];
+            hi[a] = ng;
+        } else {
+            lo[a] = 0;
+            hi[a] = 2 * ng - s[a];
+        }
+    }
+    float acc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, all[3] = {0.0f, 0.0f, 0.0f};
+    int n6[6] = {0, 0, 0, 0, 0, 0};
+    for (int l = lo[0]; l < hi[0]; l++) {
+        const int rx = ng2 * (l + s[0] - ng), lx = ng2 * l;
+        for (int m = lo[1]; m < hi[1]; m++) {
+            const int ry = rx + ng * (m + s[1] - ng), ly = lx + ng * m;
+            for (int n = lo[2]; n < hi[2]; n++) {
+                const int rz = ry + (s[2] + n - ng), lz = ly + n;
+                const float p[3] = {rec_rpsc[rz] * lig_r[lz], rec_i[rz] * lig_i[lz],
+                                    (rec_r[rz] - rec_rpsc[rz]) * lig_r[lz]};
+                for (int k = 0; k < 3; k++) {
+                    all[k] += p[k];
+                    if (p[k] < 0) {
+                        acc[2 * k + 1] += p[k];
+                        n6[2 * k + 1]++;
+                    } else if (p[k] > 0) {
+                        acc[2 * k] += p[k];
+                        n6[2 * k]++;
+                    }
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 6; k++) {
+        out[k] = acc[k];
+        if (cnt) cnt[k] = n6[k];
+    }
+    for (int k = 0; k < 3; k++) out[6 + k] = all[k];
+}

@@ -88,6 +88,13 @@ int or_dock(int ng, int nf, float gw, int na, const float *coord, const float *c
             const float *rec_r, const float *rec_i, const float *angles,
             const int *rot_ids, int n_rot, int t, int threads, or_sortscore *out);
 
+/* A15 (-O): the float sums of Docking::output_detail for one pose at grid translation
+ * trans (a decoy's index[1..3]), in the reference's order (src/docking.cpp:646-654,
+ * :720-807): out = (rPSC+, rPSC-, ELEC+, ELEC-, ACE+, ACE-, rPSC, ELEC, ACE unsplit), cnt
+ * (optional) the counts of the six split sums. */
+void or_detail_sums_f32(int ng, const float *rec_r, const float *rec_i, const float *rec_rpsc, const float *lig_r,
+                        const float *lig_i, const int trans[3], float out[9], int cnt[6]);
+
 #ifdef __cplusplus
 }
 #endif

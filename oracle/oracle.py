@@ -507,3 +507,113 @@ def rotvec_matrix(w):
     if th < 1e-12:
         return np.eye(3) + K
     return np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * (K @ K)
+
+
+# ------------------------------------------------------------------ detailed output (-O)
+def detail_sums_f32(ng, rec_r, rec_i, rec_rpsc, lig_r, lig_i, trans):
+    """The float sums of Docking::output_detail in the reference's order
+    (src/docking.cpp:646-654, :720-807): (rPSC+, rPSC-, ELEC+, ELEC-, ACE+, ACE-, rPSC,
+    ELEC, ACE unsplit) as float32, and the counts of the six split sums."""
+    a = [np.ascontiguousarray(x, np.float32).ravel() for x in (rec_r, rec_i, rec_rpsc, lig_r, lig_i)]
+    tr = np.ascontiguousarray(trans, np.int32)
+    out = np.zeros(9, np.float32)
+    cnt = np.zeros(6, np.int32)
+    lib().or_detail_sums_f32(C.c_int(ng), *[_fp(x) for x in a], _ip(tr), _fp(out), _ip(cnt))
+    return out, cnt
+
+
+def _atom_class(key):
+    """src/protein.cpp:50-60: 1 for CA, 2 for C and N, else 0."""
+    if key[:3] == "CA ":
+        return 1
+    if key[:2] in ("C ", "N "):
+        return 2
+    return 0
+
+
+def output_detail(rec, lig, ng, gw, rec_file, lig_file, decoys, angles, elec_ratio=1.0, ace_ratio=1.0):
+    """The .detail and .csv texts of Docking::output_detail (src/docking.cpp:456-898) for
+    the decoys (rotation number, tx, ty, tz) of angle table `angles`: header, the seven
+    poses around the input position with the unrotated ligand, then the decoys; float
+    arithmetic as the reference's (float sums in its order, RMSD terms squared in double
+    and accumulated in float), printed with its formats."""
+    f32 = np.float32
+    nf1 = 2 * ng
+    gwf = f32(gw)
+    pi = f32(4.0 * np.arctan(1.0))
+    gr, gi = receptor_voxel(ng, gw, rec, aceratio=ace_ratio, eratio=elec_ratio)
+    rp, _ = receptor_voxel(ng, gw, rec, aceratio=0.0, eratio=elec_ratio)
+    coordr = np.asarray(rec.center, np.float32)
+    coordl = np.asarray(lig.center, np.float32)
+    diff = (coordr - coordl).astype(np.float32)
+    cor = []
+    for i in range(3):
+        q = f32(diff[i] / gwf)
+        c = int(np.floor(abs(float(q)) + 0.5)) * (1 if q >= 0 else -1)  # round(), halves away from zero
+        cor.append(c + nf1 if c < 0 else c)
+    cls = [_atom_class(k) for k in lig.keys]
+    n_lca = sum(1 for c in cls if c == 1)
+    xyz = np.asarray(lig.xyz, np.float32)
+
+    def rmsd(tem):
+        lca = f32(0.0)
+        for i in range(lig.na):
+            if cls[i] > 0:
+                for k in range(3):
+                    moved = f32(f32(xyz[i, k] + f32(gwf * f32(tem[k] - ng))) + diff[k])
+                    d = float(f32(moved - xyz[i, k]))
+                    if cls[i] == 1:
+                        lca = f32(float(lca) + d * d)
+        r = f32(np.sqrt(f32(lca / f32(n_lca))))
+        return r, 1 if r < f32(5.0) else 0
+
+    def lig_grids(theta):
+        xyz_r = rotate_ligand(lig.xyz, lig.center, rotation_coef(np.asarray(theta, np.float32)))
+        return ligand_voxel(ng, gw, xyz_r, lig.radius, lig.charge)
+
+    fmt = "%5d: (%5.2f,%5.2f,%5.2f) (%3d,%3d,%3d) %11.2f %8.0f %8.0f %8.0f %10.2f %10.2f %7.3f %7d"
+    columns = ("Rank   Angle               Trans         total       rPSC     rPSC(+)  rPSC(-)  ELEC       RecACE"
+               "     LCARMSD NearNat")
+    d = ["%d\t%.2f" % (nf1, gw), "\t%f\t%f\t%f" % (0.0, 0.0, 0.0),
+         rec_file + "\t%f\t%f\t%f" % tuple(float(v) for v in coordr),
+         lig_file + "\t%f\t%f\t%f" % tuple(float(v) for v in coordl),
+         "\nElec score ratio: %6.2f\nACE  score ratio: %6.2f" % (elec_ratio, ace_ratio),
+         "\ncorrect position: angle = (%5.2f,%5.2f,%5.2f), trans = (%3d,%3d,%3d) \n" % (0.0, 0.0, 0.0, *cor),
+         " *** score around correct position *** \n", columns]
+    lr0, li0 = lig_grids((0.0, 0.0, 0.0))
+    c = list(cor)
+    for rank in range(7):  # src/docking.cpp:596-700
+        if rank == 1:
+            c[0] -= 1
+        if rank == 2:
+            c[0] += 2
+        if rank == 3:
+            c[0] -= 1
+            c[1] -= 1
+        if rank == 4:
+            c[1] += 2
+        if rank == 5:
+            c[1] -= 1
+            c[2] -= 1
+        if rank == 6:
+            c[2] += 2
+        s, _ = detail_sums_f32(ng, gr, gi, rp, lr0, li0, c)
+        total = f32(f32(s[6] + s[7]) + s[8])
+        tem = [(3 * ng - v) % (2 * ng) for v in c]
+        r, nn = rmsd(tem)
+        d.append(fmt % (rank + 1, 0.0, 0.0, 0.0, c[0], c[1], c[2], total, s[6], s[0], s[1], s[7], s[8], r, nn))
+    d += ["\n *** docking results *** \n", columns]
+    csv = ["Rank,Anglex,Angley,Anglez,Transi,Transj,Transk,total,rPSC,rPSC_gain,rPSC_penalty,ELEC ,RecACE,LCARMSD,NearNat"]
+    for rank, (a, tx, ty, tz) in enumerate(decoys):  # src/docking.cpp:711-857
+        theta = np.asarray(angles[a], np.float32)
+        lr, li = lig_grids(theta)
+        s, _ = detail_sums_f32(ng, gr, gi, rp, lr, li, (tx, ty, tz))
+        rpsc, elec, ace = f32(s[0] + s[1]), f32(s[2] + s[3]), f32(s[4] + s[5])
+        total = f32(f32(rpsc + elec) + ace)
+        th = [f32(float(v) - float(pi) * 2.0) if v >= pi else v for v in theta]
+        tem = [(3 * ng - v) % (2 * ng) for v in (tx, ty, tz)]
+        r, nn = rmsd(tem)
+        vals = (total, rpsc, s[0], s[1], elec, ace, r)
+        d.append(fmt % (rank + 1, *th, tx, ty, tz, *vals, nn))
+        csv.append("%d, %f,%f,%f,%d,%d,%d,%f,%f,%f,%f,%f,%f,%f,%d" % (rank + 1, *th, tx, ty, tz, *vals, nn))
+    return "\n".join(d) + "\n", "\n".join(csv) + "\n"

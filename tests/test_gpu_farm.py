@@ -1,9 +1,12 @@
 """GPU: the job-table farm (mdkh_run_table) docks every job of a table exactly as the
-single-pair program does (byte-identical .out, .detail and .csv files), with one context per FFT size
-reused across jobs (ligands of fewer atoms than the context was made for), and retries
-a failing job before it reports it."""
+single-pair program does (byte-identical .out, .detail and .csv files), with two contexts per
+FFT size reused across jobs (ligands of fewer atoms than the context was made for; one job
+set up and queued while the previous one runs), retries a failing job before it reports it,
+and keeps its host memory bounded on a long table."""
 import os
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -58,13 +61,15 @@ def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):
 
 
 def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
-    """Eight FFT sizes and then two of the first ones again: the farm keeps at most six
-    contexts alive, frees the least recently used one for each new size, and makes a
-    context again (with its rotation table) when an evicted size comes back.  Every job's
-    .out equals the single-pair program's."""
+    """Ten FFT sizes, then two of the first ones again, then one size three times in a row:
+    the farm keeps at most eight contexts alive, frees the least recently used one (never
+    the one whose job is in flight) for each new size, makes a context again (with its
+    rotation table) when an evicted size comes back, and alternates between two contexts of
+    one size for consecutive jobs of that size.  Every job's .out equals the single-pair
+    program's."""
     r = os.path.join(golden_dir, "1gcq_r.pdb")
     l = os.path.join(golden_dir, "1gcq_l.pdb")
-    sizes = [80, 84, 88, 90, 96, 98, 100, 108, 80, 88]
+    sizes = [80, 84, 88, 90, 96, 98, 100, 108, 110, 112, 80, 88, 96, 96, 96]
     lines = ["%s\t%s\t%s\t%d" % (r, l, tmp_path / ("farm_%d.out" % k), nf) for k, nf in enumerate(sizes)]
     tb = tmp_path / "jobs.table"
     tb.write_text("TITLE=sizes\nPARAM=-R $1 -L $2 -o $3 -F $4 -r 24 -N 20 -W 8\n" + "\n".join(lines) + "\n")
@@ -79,3 +84,41 @@ def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
         got = (tmp_path / ("farm_%d.out" % k)).read_bytes()
         assert got == single.read_bytes(), (k, nf)
         assert got.split(b"\t")[0] == str(nf).encode()
+
+
+def _farm_peak_rss_kib(table, log):
+    """Peak resident memory of the farm program run on `table` in a process of its own."""
+    dp = os.path.join(os.path.dirname(M.__file__), "bin", "megadock-hip-dp")
+    code = ("import resource, subprocess, sys\n"
+            "r = subprocess.run(sys.argv[1:], stdout=subprocess.DEVNULL)\n"
+            "print(r.returncode, resource.getrusage(resource.RUSAGE_CHILDREN).ru_maxrss)")
+    out = subprocess.run([sys.executable, "-c", code, dp, "-tb", str(table), "-lg", str(log), "-pj", "8"],
+                         capture_output=True, text=True, timeout=300)
+    rc, kib = out.stdout.split()
+    assert rc == "0", out.stderr
+    return int(kib)
+
+
+def test_farm_host_memory_stays_bounded_on_a_long_table(tmp_path, golden_dir):
+    """A table on which the GPU is the slower stage (the 54000-rotation set: 0.2 s of GPU
+    work per job, and 2.6 MB of rotation angles and coefficients per prepared job): the
+    preparation threads stop at two waiting jobs each, so the queue of prepared jobs never
+    holds more than that, and the farm's peak memory does not grow with the number of jobs
+    (an unbounded queue would hold every job of the longer table, about 200 MB more)."""
+    r = os.path.join(golden_dir, "1gcq_r.pdb")
+    l = os.path.join(golden_dir, "1gcq_l.pdb")
+    peak = {}
+    for n in (20, 96):
+        tb = tmp_path / ("long_%d.table" % n)
+        tb.write_text("TITLE=long\nPARAM=-R $1 -L $2 -o $3 -D -N 5\n"
+                      + "\n".join("%s\t%s\t%s" % (r, l, tmp_path / ("j%d_%d.out" % (n, k))) for k in range(n)) + "\n")
+        print("farm of %d jobs of 54000 rotations" % n, flush=True)
+        log = tmp_path / ("long_%d.log" % n)
+        peak[n] = _farm_peak_rss_kib(tb, log)
+        assert len(list(tmp_path.glob("j%d_*.out" % n))) == n
+        waiting = re.search(r"at most (\d+) \(limit (\d+)\)", log.read_text())
+        assert waiting and int(waiting.group(1)) <= int(waiting.group(2)) == 16, waiting
+        if n == 96:
+            assert int(waiting.group(1)) == 16  # the preparation ran ahead until the limit held it
+    print("peak RSS: %s KiB" % peak, flush=True)
+    assert peak[96] - peak[20] < 100 * 1024, peak

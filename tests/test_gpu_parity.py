@@ -359,8 +359,8 @@ def test_topn_device_merge_all_tied(gpu, gcq):
 
 @pytest.mark.parametrize("ng,env", [(48, {"MDK_XCORR_VARIANT": v}) for v in "01234567"] +
                          [(48, {"MDK_INV_VARIANT": v}) for v in "123"] +
-                         [(48, {"MDK_STREAMS": "1"}), (48, {"MDK_FWD_VARIANT": "1"})] +
-                         [(72, {"MDK_XCORR_VARIANT": v}) for v in "4567"] + [(72, {"MDK_FWD_VARIANT": "1"})] +
+                         [(48, {"MDK_STREAMS": "1"}), (48, {"MDK_FWD_VARIANT": "1"}), (48, {"MDK_FWD_VARIANT": "2"})] +
+                         [(72, {"MDK_XCORR_VARIANT": v}) for v in "4567"] + [(72, {"MDK_FWD_VARIANT": v}) for v in "12"] +
                          [(40, {"MDK_XCORR_VARIANT": "7"}), (128, {"MDK_XCORR_VARIANT": "7"})])
 def test_tuning_variants_give_identical_results(gpu, gcq, ng, env):
     """Every kernel variant computes the same operations in the same order: results are

@@ -74,5 +74,5 @@ def test_entry_points_refuse_invalid_arguments(golden_dir):
     rc = M.rotation_coefs(np.zeros((1, 3), np.float32))
     with pytest.raises(M.MdkError, match="translation out of range"):
         ctx.score_detail(rp, rc, np.array([[3 * ng, 0, 0]], np.int32))
-    assert ctx.score_detail(rp, rc[:0], np.zeros((0, 3), np.int32)).shape == (0, 6)
+    assert ctx.score_detail(rp, rc[:0], np.zeros((0, 3), np.int32)).shape == (0, 9)
     ctx.close()

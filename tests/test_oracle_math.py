@@ -184,3 +184,42 @@ def test_detail_sums_split_and_ace_part():
     assert d[0] == 2.0 * cnt and d[1] == 0.0        # rPSC = 1 * 2
     assert d[2] == 0.0 and d[3] == -1.0 * cnt       # ELEC = -2 * 0.5
     assert d[4] == 4.0 * cnt and d[5] == 0.0        # ACE = (3 - 1) * 2
+
+
+@pytest.mark.parametrize("t", [(0, 0, 0), (1, 2, 3), (13, 7, 9), (3, 15, 1)])
+def test_detail_float_sums_exact_on_integer_grids(t):
+    """Integer-valued grids of small magnitude: every product and every partial sum is an
+    integer below 2^24, so the reference's float sums are exact in any order and equal the
+    float64 sums; the unsplit sums equal the split halves added."""
+    ng = 8
+    rng = np.random.default_rng(sum(t))
+    rr, ri, rp, lr, li = [rng.integers(-3, 4, ng ** 3).astype(np.float32) for _ in range(5)]
+    f, cnt = O.detail_sums_f32(ng, rr, ri, rp, lr, li, t)
+    d = O.detail_sums(ng, rr, ri, rp, lr, li, t)
+    assert np.array_equal(f[:6].astype(np.float64), d), (f, d)
+    assert np.array_equal(f[6:], f[0:6:2] + f[1:6:2])
+    # the counts are the non-zero products of each sign
+    assert cnt.sum() > 0 and np.all(cnt >= 0)
+
+
+def test_detail_float_sums_follow_the_raster_order():
+    """Products 2^24 then 1, 1, 1 along the reference's order (l, m, n ascending): added one
+    by one in float each 1 is lost (2^24 + 1 rounds back to 2^24), so the sum is 2^24; any
+    order that adds the ones first gives 2^24 + 4.  One receptor voxel per product, the
+    ligand all ones, translation 0 (s = ng: voxel i meets voxel i)."""
+    ng = 4
+    rp = np.zeros(ng ** 3, np.float32)
+    cells = [(1, 2, 3), (2, 0, 1), (2, 3, 0), (3, 1, 2)]  # raster order: the big one first
+    for k, (l, m, n) in enumerate(cells):
+        rp[(l * ng + m) * ng + n] = 2.0 ** 24 if k == 0 else 1.0
+    one = np.ones(ng ** 3, np.float32)
+    zero = np.zeros(ng ** 3, np.float32)
+    f, cnt = O.detail_sums_f32(ng, rp, zero, rp, one, zero, (0, 0, 0))
+    assert f[0] == np.float32(2.0 ** 24) and f[6] == np.float32(2.0 ** 24)
+    assert cnt[0] == 4
+    # the big product last in raster order: the ones add up first
+    rp2 = np.zeros(ng ** 3, np.float32)
+    for k, (l, m, n) in enumerate(cells):
+        rp2[(l * ng + m) * ng + n] = 2.0 ** 24 if k == 3 else 1.0
+    f2, _ = O.detail_sums_f32(ng, rp2, zero, rp2, one, zero, (0, 0, 0))
+    assert f2[0] == np.float32(2.0 ** 24 + 4)
