@@ -145,6 +145,12 @@ def bench_c2(args, M, workloads):
         seq = {"seconds": round(seq_s, 3), "rotations_per_s": round(rot / seq_s, 1),
                "farm_speedup": round(seq_s / farm_s, 2), "outputs_identical": same}
     spec = set(M.TUNED_NF + M.FURTHER_NF)
+    # the yardstick summed over the pairs: B_rot = 32 nf^3 bytes per rotation of each pair
+    alg = sum(3600 * 32.0 * n ** 3 for n in nfs)
+    roof = {"bound": "hbm", "achieved": round(alg / farm_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / farm_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit_of_work": "the whole table: sum over its pairs of 3600 rotations x 32 nf^3 bytes, over the farm's "
+                            "wall time (preparation, transfers, merges and .out files included)"}
     print(json.dumps({
         "metric": "c2 job table: rotations/s over receptor/ligand pairs of uneven FFT size",
         "value": round(rot / farm_s, 1), "unit": "rotations/s", "n_gpus": 1, "higher_is_better": True,
@@ -152,6 +158,7 @@ def bench_c2(args, M, workloads):
         "config": {"workload": "c2: first %d receptors x %d ligands of the 32 x 32 table, auto grid, 3600 rotations, t = 1"
                    % (nr, nl), "pairs": len(pairs), "nf": {str(n): nfs.count(n) for n in sorted(set(nfs))},
                    "pairs_on_specialised_kernels": sum(1 for n in nfs if n in spec)},
+        "roofline": roof,
         "farm_seconds": round(farm_s, 3), "pairs_per_s": round(len(pairs) / farm_s, 2),
         "one_after_another": seq,
     }), flush=True)
@@ -301,7 +308,7 @@ def main():
     tuned = None
     if not args.no_autotune and not args.sweep:
         tuned = ctx.autotune(0, min(R, 16 * B))
-        log("autotune: correlation variant %d, inverse variant %d" % tuned)
+        log("autotune: correlation variant %d, inverse variant %d, forward variant %d" % tuned)
 
     if args.sweep:
         nbytes = 1 << 30
@@ -363,7 +370,7 @@ def main():
     # its own, not shared with the other stream's kernels (reported beside, not timed above)
     env = {"MDK_STREAMS": "1"}
     if tuned:  # the variants the timed passes ran
-        env.update(MDK_XCORR_VARIANT=str(tuned[0]), MDK_INV_VARIANT=str(tuned[1]))
+        env.update(MDK_XCORR_VARIANT=str(tuned[0]), MDK_INV_VARIANT=str(tuned[1]), MDK_FWD_VARIANT=str(tuned[2]))
     saved = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     octx = M.Context(nf, ng, lig.na, t=t, max_batch=B)
@@ -437,7 +444,8 @@ def main():
                                % (args.workload, cfg["rec_res"], rec.na, cfg["lig_res"], lig.na, nf, R, t, cfg["n_out"]),
                    "nf": nf, "ng": ng, "rotations": R, "t": t, "decoys": n_out, "batch": B,
                    "step": "dock all rotations + device merge of the best decoys",
-                   "variants": {"correlation": tuned[0], "inverse": tuned[1], "chosen_by": "autotune"} if tuned
+                   "variants": {"correlation": tuned[0], "inverse": tuned[1], "forward": tuned[2],
+                                "chosen_by": "autotune"} if tuned
                    else "defaults"},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -175,11 +175,15 @@ int mdk_voxelize(mdk_ctx *ctx, const float *rotcoef, int B, float *grid_r, float
  * rotation, i.e. Re IFFT(conj(R) * FFT(ligand)). */
 int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
 
-/* Autotuner: times every kernel variant of the specialised pipeline (correlation-pass
- * tiling and receptor reuse, fused-inverse loading) on rotations [first, first+count) of
- * the table and keeps the fastest for this context; choice[0..1] receives the chosen
- * (correlation, inverse) variants.  All variants give bitwise-identical results.  No
- * reference counterpart. */
+/* Autotuner: times the kernel variants built for this context's FFT size (correlation-pass
+ * tiling and receptor reuse and fused-inverse loading together, then the forward pass's
+ * prefetch or split) on rotations [first, first+count) of the table and keeps the fastest
+ * for this context; choice[0..2] receives the chosen (correlation, inverse, forward)
+ * variants.  The variants give bitwise-identical results, except at nf = 256, where the
+ * two-pass forms (inverse variant 0, forward variant 1) round differently from the
+ * half-plane fused passes.  The sizes 80, 96 and 144 have every variant, 256 correlation
+ * 0-7 (6 with the fused forward), inverse 0-3 and forward 0-1, the other specialised sizes
+ * correlation 5-7, inverse 2-3 and forward 0 and 2.  No reference counterpart. */
 int mdk_autotune(mdk_ctx *ctx, int first, int count, int *choice);
 
 /* Off-grid score of B poses with its translation gradient, for refining decoys.  No

@@ -221,10 +221,10 @@ class Context:
 
     def autotune(self, first, count):
         """Time the kernel variants on rotations [first, first+count) of the table and keep
-        the fastest; returns the chosen (correlation, inverse) variant numbers."""
-        ch = (C.c_int * 2)()
+        the fastest; returns the chosen (correlation, inverse, forward) variant numbers."""
+        ch = (C.c_int * 3)()
         _check(lib().mdk_autotune(self.h, C.c_int(first), C.c_int(count), ch), "mdk_autotune")
-        return ch[0], ch[1]
+        return ch[0], ch[1], ch[2]
 
     def score_spectral(self, rotcoef, t):
         """(B, 4): band-limited score at continuous translations t (B, 3) in voxels and

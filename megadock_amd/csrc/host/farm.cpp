@@ -13,12 +13,14 @@
 //     PDB reading, grid sizing, the receptor's shape/electrostatics grids (cached per
 //     receptor file and grid, since a table repeats receptors), rotation coefficients;
 //   the calling thread docks the prepared jobs on GPU 0 through libmdk contexts kept per
-//     FFT size, two per size (re-created only when a job needs more ligand atoms, another t
-//     or another batch; at most eight alive, the least recently used freed first), taking
-//     first a prepared job whose size has a live context.  One job is in flight while the
-//     next is set up: the next job's receptor, ligand and batches are queued on its own
-//     context before the GPU thread waits for the previous job's merge and writes its .out
-//     file, so the GPU runs one job's batches straight after the other's.
+//     FFT size (re-created only when a job needs more ligand atoms, another t or another
+//     batch; at most six alive, the least recently used freed first), taking first a
+//     prepared job whose size has a live context and differs from the job in flight.  One
+//     job is in flight while the next is set up: the next job's receptor, ligand and
+//     batches are queued on its own context before the GPU thread waits for the previous
+//     job's merge and writes its .out file, so the GPU runs one job's batches straight
+//     after the other's (a job of the in-flight job's size waits for it, as they share a
+//     context).
 // Host memory stays bounded on long tables: at most two prepared jobs per preparation
 // thread wait for the GPU, and the receptor grid cache keeps the grids of the last 16
 // receptors (and of any receptor a waiting job still uses).
@@ -317,42 +319,33 @@ struct Slot {
     long last_use = 0;
 };
 
-// Two contexts per FFT size, so that a job can be set up and queued on one while the job
-// in flight still runs on the other; at most kMaxLive at a time (each holds up to two
-// batches of about 8 GiB of buffers): a table with more sizes than that frees the least
-// recently used context (never the busy one) before it creates another.
+// One context per FFT size, at most kMaxLive at a time (each holds up to two batches of
+// about 8 GiB of buffers): a table with more sizes than that frees the least recently used
+// context (never the busy one, whose job is in flight) before it creates another.
 class Contexts {
 public:
-    static constexpr int kMaxLive = 8;
+    static constexpr int kMaxLive = 6;
     ~Contexts() {
         for (auto &kv : slots_)
             if (kv.second.ctx) mdk_destroy(kv.second.ctx);
     }
-    // a context for the job's FFT size, other than `busy`, able to hold its ligand, t and
-    // batch (its slot: the context and the rotation set loaded into it)
+    // the context of the job's FFT size, able to hold its ligand, t and batch (its slot: the
+    // context and the rotation set loaded into it); `busy` (the in-flight job's context,
+    // another size) is never freed
     Slot *get(const Job &j, mdk_ctx *busy, std::string &err) {
         const int nf = 2 * j.ng, na = j.lig->num_atoms(), t = j.prm.num_sort;
         int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
         batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
-        // of the two slots of the size: a live one that is not busy and fits, else a free one
-        Slot *s = nullptr;
-        for (int k = 0; k < 2 && !s; k++) {
-            Slot &c = slots_[{nf, k}];
-            if (c.ctx && c.ctx != busy && c.na >= na && c.t == t && c.batch >= batch) s = &c;
-        }
-        for (int k = 0; k < 2 && !s; k++) {
-            Slot &c = slots_[{nf, k}];
-            if (c.ctx != busy || !busy) s = &c;
-        }
+        Slot *s = &slots_[nf];
         s->last_use = ++clock_;
         if (s->ctx && (s->na < na || s->t != t || s->batch < batch)) {
             mdk_destroy(s->ctx);
             *s = Slot();
         }
         if (!s->ctx) {
-            free_others(kMaxLive - 1, busy);
+            free_others(kMaxLive - 1, nf, busy);
             int rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);
-            if (rc && free_others(0, busy) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);  // out of memory?
+            if (rc && free_others(0, nf, busy) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);  // out of memory?
             if (rc) {
                 err = std::string("mdk_create: ") + mdk_last_error();
                 *s = Slot();
@@ -366,23 +359,20 @@ public:
         return s;
     }
     bool has(int nf) const {
-        for (int k = 0; k < 2; k++) {
-            auto it = slots_.find({nf, k});
-            if (it != slots_.end() && it->second.ctx) return true;
-        }
-        return false;
+        auto it = slots_.find(nf);
+        return it != slots_.end() && it->second.ctx != nullptr;
     }
 
 private:
-    // destroy the least recently used contexts other than `busy` until at most `keep` of
-    // them are left; returns how many were destroyed
-    int free_others(int keep, mdk_ctx *busy) {
+    // destroy the least recently used contexts other than size `keep_nf` and `busy` until
+    // at most `keep` of them are left; returns how many were destroyed
+    int free_others(int keep, int keep_nf, mdk_ctx *busy) {
         int freed = 0;
         for (;;) {
             int n = 0;
             Slot *lru = nullptr;
             for (auto &kv : slots_)
-                if (kv.second.ctx && kv.second.ctx != busy) {
+                if (kv.first != keep_nf && kv.second.ctx && kv.second.ctx != busy) {
                     n++;
                     if (!lru || kv.second.last_use < lru->last_use) lru = &kv.second;
                 }
@@ -392,7 +382,7 @@ private:
             freed++;
         }
     }
-    std::map<std::pair<int, int>, Slot> slots_;  // (nf, 0 or 1)
+    std::map<int, Slot> slots_;
     long clock_ = 0;
 };
 
@@ -657,15 +647,21 @@ int run_table(int argc, char **argv, std::string &err) {
             if (ok) written = std::move(inflight);
             else report(std::move(inflight), false);
         };
-        // a prepared job whose FFT size has a live context goes first: fewer contexts made
-        // and freed when the table mixes more sizes than stay alive
+        // a prepared job whose FFT size has a live context goes first (fewer contexts made
+        // and freed when the table mixes more sizes than stay alive), best of another size
+        // than the job in flight, whose batches it can then queue behind
+        auto live_other = [&](const std::unique_ptr<Job> &x) {
+            return cs.has(2 * x->ng) && (!inflight || x->ng != inflight->ng);
+        };
         auto live = [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); };
         for (;;) {
-            if (!ready.try_pop_prefer(j, live)) {
+            if (!ready.try_pop_prefer(j, live_other)) {
                 complete_inflight();  // nothing prepared: finish what is queued before waiting
                 write_pending();
                 if (!ready.pop_prefer(j, live)) break;
             }
+            // a job of the in-flight job's size shares its context: that job completes first
+            if (inflight && inflight->ng == j->ng) complete_inflight();
             bool ok = false;
             try {
                 ok = issue(*j, cs, inflight ? inflight->ctx : nullptr) == 0;  // queued behind the job in flight

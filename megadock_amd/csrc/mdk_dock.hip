@@ -452,7 +452,7 @@ struct mdk_ctx {
     int xcorr_variant = 6;    // MDK_XCORR_VARIANT: correlation-pass variant (tuning)
     int cu_count = 256;       // compute units (persistent grids)
     int inv_variant = 2;      // MDK_INV_VARIANT: fused-inverse variant (tuning)
-    int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, 1 separate passes, 2 fused, prefetching
+    int fwd_variant = 0;      // MDK_FWD_VARIANT: 0 fused forward z + y, prefetching (nf = 256: over half planes), 1 separate passes, 2 fused, not prefetching
     float2 *d_rec = nullptr;
     bool have_rec = false, have_lig = false;
     // ligand
@@ -752,19 +752,16 @@ int mdk_create(int device, int nf, int ng, int na, int t, int max_batch, mdk_ctx
             return bail(fail("mdk_create: twiddle upload failed"));
         const char *g = std::getenv("MDK_FORCE_GENERIC");
         c->use_pipe = pipe_available(nf) && !(g && g[0] == '1');
-        // default variants, as measured per tuned size (the further sizes build only 6 and 2):
-        // 96^3 takes correlation variant 7 (no padding columns; variant 6's last chunk of 16
-        // columns holds one: 164.9-165.7k against 160.3-163.0k rotations/s) and inverse
-        // variant 3 (167.7-169.1k against 164.2-165.4k); 80^3 takes inverse variant 3 as well
-        // (293.3-293.8k against 280.0-280.5k; correlation 7 is slower there); 144^3 keeps 6
-        // and 2 (inverse variant 3: 58.8-59.1k against 59.3-59.4k)
-        const int dx = nf == 96 ? 7 : 6, di = nf == 96 || nf == 80 ? 3 : 2;
+        // default variants per size (pipe_defaults: measured on MI355X), unless the
+        // environment asks for others
+        int dx = 6, di = 2, df = 0;
+        (void)pipe_defaults(nf, &dx, &di, &df);
         const char *xv = std::getenv("MDK_XCORR_VARIANT");
         c->xcorr_variant = xv ? std::atoi(xv) : dx;
         const char *iv = std::getenv("MDK_INV_VARIANT");
         c->inv_variant = iv ? std::atoi(iv) : di;
         const char *fv = std::getenv("MDK_FWD_VARIANT");
-        c->fwd_variant = fv ? std::atoi(fv) : 0;
+        c->fwd_variant = fv ? std::atoi(fv) : df;
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
             c->cu_count = cus;
@@ -1242,17 +1239,24 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
     if (first < 0 || count < 1 || first + count > c->nrot) return fail("mdk_autotune: range outside the rotation table");
     HIPCHK(hipSetDevice(c->device));
     if (c->use_pipe) {
-        // every variant computes the same results (same operations in the same order), so the
-        // fastest one can be chosen freely; each is timed twice after one warm-up pass
+        // the variants built for this size: correlation x inverse first, then the forward
+        // pass; each timed twice after one warm-up pass.  Every variant computes the same
+        // results (same operations in the same order), except at nf = 256, where the two-pass
+        // forms (inverse variant 0, forward variant 1) and the half-plane fused passes round
+        // differently
+        auto time_it = [&](float &ms) {
+            drop_graphs(c);
+            return mdk_dock_range(c, first, count) || mdk_time_range(c, first, count, 2, &ms);
+        };
         float best = 0.f;
-        int bx = c->xcorr_variant, bi = c->inv_variant;
+        int bx = c->xcorr_variant, bi = c->inv_variant, bf = c->fwd_variant;
         for (int xv = 3; xv <= 7; xv++)  // 0..2 are far slower on every size measured
             for (int iv = 0; iv <= 3; iv++) {
+                if (!pipe_has_variant(c->nf, 0, xv) || !pipe_has_variant(c->nf, 1, iv)) continue;
                 c->xcorr_variant = xv;
                 c->inv_variant = iv;
-                drop_graphs(c);
                 float ms = 0.f;
-                if (mdk_dock_range(c, first, count) || mdk_time_range(c, first, count, 2, &ms)) return 1;
+                if (time_it(ms)) return 1;
                 if (best == 0.f || ms < best) {
                     best = ms;
                     bx = xv;
@@ -1261,11 +1265,25 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
             }
         c->xcorr_variant = bx;
         c->inv_variant = bi;
+        for (int fv : {0, 1, 2}) {
+            if (!pipe_has_variant(c->nf, 2, fv) || fv == bf) continue;
+            const int keep = c->fwd_variant;
+            c->fwd_variant = fv;
+            float ms = 0.f;
+            if (time_it(ms)) return 1;
+            if (ms < best) {
+                best = ms;
+                bf = fv;
+            }
+            c->fwd_variant = keep;
+        }
+        c->fwd_variant = bf;
         drop_graphs(c);
     }
     if (choice) {
         choice[0] = c->xcorr_variant;
         choice[1] = c->inv_variant;
+        choice[2] = c->fwd_variant;
     }
     return 0;
 }

@@ -283,11 +283,17 @@ struct PipeArgs {
     int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches, 2 and 3 as 0 and 1
                         // with running top-t lists over contiguous planes (MDK_INV_VARIANT)
     bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
-    int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, 1 separate, 2 fused with the next plane prefetched (MDK_FWD_VARIANT)
+    int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, the next plane prefetched (nf = 256: over half planes), 1 separate, 2 fused without the prefetch (MDK_FWD_VARIANT)
 };
 
 // Compile-time specialised pipeline (mdk_pipe.hip) for the grid sizes it was built for.
 bool pipe_available(int nf);
+// default kernel variants of a size (correlation, inverse, forward); false without a
+// specialised pipeline
+bool pipe_defaults(int nf, int *xcorr, int *inv, int *fwd);
+// whether a size's specialised pipeline was built with variant v of a pass (which: 0
+// correlation, 1 inverse, 2 forward)
+bool pipe_has_variant(int nf, int which, int v);
 // top-t candidate lists the specialised pipeline leaves per rotation (each of t entries)
 int pipe_cands_per_rotation(int nf);
 hipError_t pipe_run(int nf, const PipeArgs &a);

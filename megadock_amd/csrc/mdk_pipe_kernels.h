@@ -234,6 +234,165 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
     }
 }
 
+// ------------------------------------------------------------------ forward z + y, half planes
+// The fused forward pass for sizes whose z-transformed plane does not fit one workgroup's
+// LDS (nf = 256).  With M = nf/2 and W = exp(-2 pi i / nf), the kz = 2m + h outputs of a
+// z row a (non-zero at z in [nd, nd + ng), nd = nf/4, ng = M) are the M-point DFT of
+//   z'_h[j] = (a[j] + (-1)^h a[j + M]) W^(h j),   j < M,
+// where one of the two terms is zero: z'_h[j] = r[j - nd] W^(hj) for j >= nd and
+// (-1)^h r[j + nd] W^(hj) for j < nd (r: the row's voxels).  So per (rotation, x) plane and
+// parity h the z outputs form an M x ng half-plane A_h[m][y] in LDS.  Each of its columns
+// m then goes through the y transform the same way, split by the parity g of ky: both
+// halves' step-1 inputs come from the same column values, the g = 1 step-1 results wait in
+// registers while g = 0 uses the column's own storage as its exchange, and each step 2
+// writes its outputs straight to HBM.  The output is d_a2 with the kz of each run of 16 in
+// parity-split order: kz = 2m + h sits at 16 (m / 8) + 8 h + m % 8 of its row (half_slot),
+// so the 16 columns kz = 16c .. 16c + 15 that a correlation workgroup reads still fill one
+// 128-byte line (kp_x_corr reads it with PERM).
+__device__ __forceinline__ int half_slot(int h, int m) { return ((m >> 3) << 4) + (h << 3) + (m & 7); }
+__device__ __forceinline__ int half_slot_of_kz(int kz) { return half_slot(kz & 1, kz >> 1); }
+
+template <int N> constexpr bool fwd_half_fits() {
+    constexpr int M = N / 2;
+    return N % 4 == 0 && (M & (M - 1)) == 0 && M * Geo<M>::N2 == 1024 && Geo<M>::N1 * 64 == 1024 &&
+           (size_t)M * (M + 1) * 8 + (size_t)N * 8 <= 150 * 1024;
+}
+
+template <int N>
+__global__ void __launch_bounds__(1024) kp_fwd_half(VoxGrid vg, float2 *__restrict__ a2, const float2 *__restrict__ twg,
+                                                     int nplanes) {
+    constexpr int NG = N / 2, ND = N / 4, M = N / 2;
+    constexpr int N1 = Geo<M>::N1, N2 = Geo<M>::N2;  // M-point transforms: step 1 of N1 points
+    constexpr int LA = NG + 1;                        // odd row stride of A[m][y]
+    constexpr int RW = (NG + 31) / 32;
+    static_assert(fwd_half_fits<N>(), "half-plane forward does not fit");
+    static_assert(32 % N2 == 0 && ND % 32 == 0, "a z task's mask words are fixed at compile time");
+    __shared__ float2 A[M * LA];
+    __shared__ float2 tw[N];
+    for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
+    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {  // uniform across the block
+        const int b = p / NG, x = p - b * NG;
+        float2 *dst = a2 + (size_t)p * N * N;  // a2[b][x]: [ky][h M + m]
+#pragma unroll 1
+        for (int h = 0; h < 2; h++) {  // uniform
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));  // the thread's addresses are made anew in each half
+            __syncthreads();  // A free
+            // ---- z: row y, step-1 index u; the row's voxel values from the mask bits and the
+            // charges (each non-zero charge cleared at its second read, h = 1)
+            {
+                const int y = tid % NG, u = tid / NG;  // NG rows x N2 tasks = 1024
+                const int row = x * NG + y;
+                const uint32_t *dr = vg.dmo + (size_t)b * 2 * vg.mw + (size_t)row * RW;
+                float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)row * NG;
+                const float delta = vg.delta;
+                uint32_t dw[RW], ow[RW];  // the row's mask words
+#pragma unroll
+                for (int w = 0; w < RW; w++) {
+                    dw[w] = dr[w];
+                    ow[w] = dr[vg.mw + w];
+                }
+                float2 v[N1];
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    // j = u + N2 j1 with u < N2: j < ND and the mask word of k are fixed by j1
+                    constexpr bool lo = N2 * j1 < ND;
+                    constexpr int kw = (N2 * j1 + (lo ? ND : -ND)) / 32;
+                    const int j = u + N2 * j1;
+                    const int k = lo ? j + ND : j - ND;  // the voxel r[k] behind z'_h[j]
+                    const uint32_t bit = 1u << (k & 31);
+                    const float q = cr[k];
+                    if (h == 1 && __float_as_uint(q) != 0u) cr[k] = 0.0f;  // keep the charge grid zero
+                    float2 val = make_float2((dw[kw] & bit) ? delta : ((ow[kw] & bit) ? 1.0f : 0.0f), q);
+                    if (h == 1) {
+                        if (lo) val = make_float2(-val.x, -val.y);
+                        const float2 w = tw[j];  // W^j
+                        val = make_float2(val.x * w.x - val.y * w.y, val.x * w.y + val.y * w.x);
+                    }
+                    v[j1] = val;
+                });
+                rf::step1_g<M, false>(v, A, tw, 2, LA, 1, y, u);
+            }
+            __syncthreads();
+            // step 2 in two rounds of NG/2 rows, each read and then written back in place as
+            // A[m][y] (m = k1 + N1 k2); rows of one round share no storage with the other's
+#pragma unroll 1
+            for (int r = 0; r < 2; r++) {  // uniform
+                const int y = r * (NG / 2) + tid % (NG / 2), u = tid / (NG / 2);  // N1 tasks a row
+                float2 w[N2];
+                rf::step2_g<M, false>(w, A, LA, 1, y, u);
+                __syncthreads();
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    A[(u + N1 * k2) * LA + y] = w[k2];
+                });
+                __syncthreads();
+            }
+            // ---- y: column m (its ng values A[m][0..ng)), step-1 index j2, both parities g of ky
+            {
+                const int m = tid % M, j2 = tid / M;  // M columns x N2 tasks = 1024
+                // g = 1's step-1 DFT first, its results kept in registers until g = 0 is done;
+                // then g = 0's values, read again (the column is still intact)
+                float2 v1[N1];
+                sfor<0, N1>([&](auto jc) {
+                    constexpr int j1 = decltype(jc)::value;
+                    const int j = j2 + N2 * j1;
+                    const float2 a = A[m * LA + (j >= ND ? j - ND : j + ND)];
+                    const float2 sa = j < ND ? make_float2(-a.x, -a.y) : a;
+                    const float2 w = tw[j];
+                    v1[j1] = make_float2(sa.x * w.x - sa.y * w.y, sa.x * w.y + sa.y * w.x);
+                });
+                rf::RDft<N1, false>::run(v1);
+                {
+                    float2 v0[N1];
+                    sfor<0, N1>([&](auto jc) {
+                        constexpr int j1 = decltype(jc)::value;
+                        const int j = j2 + N2 * j1;
+                        v0[j1] = A[m * LA + (j >= ND ? j - ND : j + ND)];
+                    });
+                    __syncthreads();  // every column read before the exchange overwrites it
+                    rf::step1_g<M, false>(v0, A, tw, 2, 1, LA, m, j2);
+                }
+                __syncthreads();
+#pragma unroll 1
+                for (int r = 0; r < 2; r++) {  // uniform; step 2 of g = 0 in two rounds of M/2 columns
+                    const int mc = r * (M / 2) + tid % (M / 2), k1 = tid / (M / 2);
+                    float2 w[N2];
+                    rf::step2_g<M, false>(w, A, 1, LA, mc, k1);
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const int ky = 2 * (k1 + N1 * k2);
+                        st_at(dst, 8u * (unsigned)(ky * N + half_slot(h, mc)), w[k2]);
+                    });
+                }
+                __syncthreads();  // g = 0 has read the exchange
+                // the twiddled g = 1 step-1 results into the exchange
+                sfor<0, N1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    float2 xv = v1[k1];
+                    if constexpr (k1 != 0) {
+                        const float2 w = tw[2 * j2 * k1];
+                        xv = make_float2(xv.x * w.x - xv.y * w.y, xv.x * w.y + xv.y * w.x);
+                    }
+                    A[m * LA + k1 * N2 + j2] = xv;
+                });
+                __syncthreads();
+#pragma unroll 1
+                for (int r = 0; r < 2; r++) {  // uniform
+                    const int mc = r * (M / 2) + tid % (M / 2), k1 = tid / (M / 2);
+                    float2 w[N2];
+                    rf::step2_g<M, false>(w, A, 1, LA, mc, k1);
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const int ky = 2 * (k1 + N1 * k2) + 1;
+                        st_at(dst, 8u * (unsigned)(ky * N + half_slot(h, mc)), w[k2]);
+                    });
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ x + correlation
 // block (kz chunk c, ky, rotation group).  Lines 0..L-1: (ky, kz = c L + l); lines
 // L..2L-1: partner columns (-ky, -kz).  Forward x-FFT of all 2L lines, the Hermitian
@@ -248,8 +407,9 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
 // of the columns kz >= KZ0 = L floor((nf/2 + 1) / L) (the ones that do not fill a whole
 // chunk) over all ky; the chunked launch covers kz < KZ0, so no padding column is
 // transformed.
+// PERM: d_a2 holds each run of 16 kz in parity-split order (kp_fwd_half, half_slot).
 template <int N, int L, int RB, bool CACHE_R, bool B_FAST = false, bool QA = false, bool SPEC = false,
-          bool FLAT = false>
+          bool FLAT = false, bool PERM = false>
 __global__ void __launch_bounds__(2 * L *Geo<N>::T)
     kp_x_corr(const float2 *__restrict__ a2, const float2 *__restrict__ rec, float2 *__restrict__ cc,
               const float2 *__restrict__ twg, int H, int Hp, int B) {
@@ -317,7 +477,8 @@ __global__ void __launch_bounds__(2 * L *Geo<N>::T)
         __syncthreads();  // S, Q free (previous rotation done)
         if (u < N2) {
             float2 v[N1];
-            const unsigned o0 = 8u * (unsigned)((u - ND) * N * N + kyl * N + kzs);  // used only when x is in range
+            const int kzp = PERM ? half_slot_of_kz(kzs) : kzs;  // kz's place in a row of d_a2
+            const unsigned o0 = 8u * (unsigned)((u - ND) * N * N + kyl * N + kzp);  // used only when x is in range
             sfor<0, N1>([&](auto jc) {
                 constexpr int j1 = decltype(jc)::value;
                 const int x = u + N2 * j1 - ND;
@@ -782,14 +943,188 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
-template <int N, int TT, bool FULL> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
+// ------------------------------------------------------------------ half-plane inverse
+// The fused inverse for sizes whose half-spectrum plane does not fit one workgroup's LDS
+// (nf = 256): each (rotation, x) plane in two halves, its even and its odd rows y.  With
+// M = nf/2 and w = exp(+2 pi i / nf), the rows y = 2m + h of the y-IFFT of a column X are
+// the M-point IFFT of
+//   X[k] + X[k + M]              (h = 0)
+//   (X[k] - X[k + M]) w^k        (h = 1),   k < M,
+// so a half needs an M x (nf/2 + 1) half-plane in LDS.  Each half then goes through the
+// same steps as kp_inv_yz_top (complex-to-real pre-step and M-point z-IFFT per row, running
+// top-t over a workgroup's contiguous planes), and the plane is read from HBM twice instead
+// of being transformed in place there (kp_inv_y) and read again (kp_inv_z_top).
+template <int N> constexpr bool half_inverse_fits() {
+    constexpr int M = N / 2;
+    return N % 4 == 0 && (size_t)M * plane_stride(N) * 8 + (size_t)N * 8 + 16 * 16 * 8 <= 150 * 1024 &&
+           Geo<M>::N1 <= 32;
+}
+
+template <int N, int TT>
+__global__ void __launch_bounds__(1024)
+    kp_inv_yz_half(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
+                   const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
+    constexpr int NTH = 1024, NW = NTH / 64;
+    constexpr int M = N / 2, H = M + 1;
+    constexpr int LSP = plane_stride(N);  // row stride of the half-plane (odd)
+    constexpr int N1 = Geo<M>::N1, N2 = Geo<M>::N2;  // both transforms are M-point
+    // the y transform runs in rounds of whole columns, one task per thread: step 1 over
+    // CS1 = NTH / N2 columns a round, step 2 over CS2 = NTH / N1 columns a round (each round's
+    // columns are read, then written back in place, so no thread holds two tasks' outputs);
+    // the z transform over RS = NTH / N1 rows a round, its outputs pushed round by round
+    constexpr int CS1 = NTH / N2, CS2 = NTH / N1, RS = NTH / N1;
+    constexpr int R1 = (H + CS1 - 1) / CS1, R2 = (H + CS2 - 1) / CS2, RZ = M / RS;
+    static_assert(half_inverse_fits<N>(), "half-plane inverse does not fit");
+    static_assert(NTH % N1 == 0 && NTH % N2 == 0 && M % RS == 0 && M * N2 <= NTH, "whole columns and rows a round");
+    __shared__ float2 P[M * LSP];
+    __shared__ float2 tw[N];
+    __shared__ Cand wl[NW * TT];
+    for (int q = threadIdx.x; q < N; q += NTH) tw[q] = twg[q];
+    // a contiguous range of planes per workgroup; the planes of one rotation in it form a
+    // segment whose lane lists and running threshold carry over (as kp_inv_yz_top, SEG)
+    const int chunk = (nplanes + gridDim.x - 1) / gridDim.x;
+    const int p_first = blockIdx.x * chunk, p_end = min(nplanes, p_first + chunk);
+    constexpr bool QUOT = TT > 1;
+    TopR<TT> tl;
+    tl.init();
+    float run_tau = -INFINITY, cut = -INFINITY;
+    int seg0 = p_first, units = 0;  // units: z rounds of the segment done so far
+    for (int p = p_first; p < p_end; p++) {  // uniform across the block
+        const int b = p / N, x = p - b * N;
+        const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
+        for (int h = 0; h < 2; h++) {  // uniform
+            // opaque thread index: keeps the per-thread addresses out of the loops
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            __syncthreads();  // P, tw and wl are free
+            // ---- M-point y-IFFT of the H columns (line = kz, element = m), folded inputs;
+            // thread (column yl = r CS1 + tid % CS1, index yu = tid / CS1) in round r
+#pragma unroll 1
+            for (int r = 0; r < R1; r++) {  // uniform
+                const int yl = r * CS1 + tid % CS1, yu = tid / CS1;
+                if (yl < H) {
+                    float2 v[N1];
+                    sfor<0, N1>([&](auto jc) {
+                        constexpr int j1 = decltype(jc)::value;
+                        const int k = yu + N2 * j1;
+                        const float2 a = src[(unsigned)(k * Hp + yl)], c = src[(unsigned)((k + M) * Hp + yl)];
+                        if (h == 0) {
+                            v[j1] = make_float2(a.x + c.x, a.y + c.y);
+                        } else {
+                            const float2 d = make_float2(a.x - c.x, a.y - c.y), w = tw[k];  // w^k = conj(tw[k])
+                            v[j1] = make_float2(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+                        }
+                    });
+                    rf::step1_g<M, true>(v, P, tw, 2, LSP, 1, yl, yu);
+                }
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int r = 0; r < R2; r++) {  // uniform
+                const int yl = r * CS2 + tid % CS2, yu = tid / CS2;
+                float2 y[N2];
+                if (yl < H) rf::step2_g<M, true>(y, P, LSP, 1, yl, yu);
+                __syncthreads();  // the round's columns are read before they are written back
+                if (yl < H)
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        P[(yu + N1 * k2) * LSP + yl] = y[k2];
+                    });
+                __syncthreads();
+            }
+            // ---- M-point inverse of each of the M rows with the complex-to-real pre-step:
+            // Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])), w^k = conj(tw[k])
+            {
+                const int line = tid % M, u = tid / M;  // step 1: M rows x N2 tasks, one round
+                float2 v[N1];
+                if (u < N2)
+                    sfor<0, N1>([&](auto jc) {
+                        constexpr int j1 = decltype(jc)::value;
+                        const int k = u + N2 * j1;
+                        const float2 a = P[line * LSP + k], c = P[line * LSP + M - k], w = tw[k];
+                        const float2 s1 = make_float2(a.x + c.x, a.y - c.y);
+                        const float2 d1 = make_float2(a.x - c.x, a.y + c.y);
+                        const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
+                        v[j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
+                    });
+                __syncthreads();
+                if (u < N2) rf::step1_g<M, true>(v, P, tw, 2, 1, LSP, line, u);
+                __syncthreads();
+            }
+            // step 2 and the top-t, RS rows a round: row `line` of the half is y = 2 line + h;
+            // z = 2m and 2m+1 with m = u + N1 k2
+#pragma unroll 1
+            for (int r = 0; r < RZ; r++, units++) {  // uniform
+                const int line = r * RS + tid % RS, u = tid / RS;
+                float2 y[N2];
+                rf::step2_g<M, true>(y, P, 1, LSP, line, u);
+                if (smap) {  // uniform
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        const size_t o = (size_t)b * N * N * N + (size_t)(x * N + 2 * line + h) * N + 2 * (u + N1 * k2);
+                        smap[o] = y[k2].x;
+                        smap[o + 1] = y[k2].y;
+                    });
+                    continue;
+                }
+                // the running threshold of the segment, raised on its first four rounds and
+                // every eighth after (the t scores behind it are pushed when it is raised)
+                if (units < 4 || (units & 7) == 0) {  // uniform
+                    float mx = -INFINITY;
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        mx = fmaxf(mx, fmaxf(y[k2].x, y[k2].y));
+                    });
+                    const float tau = wave_kth_max(mx, t);
+                    if (tau > run_tau) {  // uniform
+                        run_tau = tau;
+                        cut = QUOT ? quotient_cut(tau, nf3f) : tau;
+                    }
+                }
+                const int base = (x * N + 2 * line + h) * N + 2 * u;
+                sfor<0, N2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const float v0 = y[k2].x, v1 = y[k2].y;
+                    if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, base + 2 * N1 * k2);
+                    if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, base + 2 * N1 * k2 + 1);
+                });
+            }
+        }
+        if (smap) continue;  // uniform
+        if (p != seg0 && (int)threadIdx.x < t) cand[(size_t)p * t + threadIdx.x] = Cand{-INFINITY, kNoIndex};  // empty list
+        if (p + 1 == p_end || (p + 1) / N != b) {  // uniform: the segment ends with this plane
+            block_top<TT, NW>(tl, t, wl, cand + (size_t)seg0 * t);
+            tl.init();
+            run_tau = cut = -INFINITY;
+            seg0 = p + 1;
+            units = 0;
+        }
+    }
+}
+
+template <int N, int TT> void launch_inv_half(const PipeArgs &a, hipStream_t s) {
     const int nplanes = a.B * N;
     const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
-    if constexpr (!FULL) {
-        // the further sizes: running lists, with the next plane prefetched (variant 3) up to
-        // nf = 128 where a thread holds one y task, as measured best at 80^3 and 96^3
-        constexpr bool PF = N <= 128 && (N / 2 + 1) * Geo<N>::N1 <= 1024 && (N / 2 + 1) * Geo<N>::N2 <= 1024;
-        kp_inv_yz_top<N, TT, PF, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    kp_inv_yz_half<N, TT><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+}
+
+// the fused inverse can prefetch the next plane (variants 1, 3) where a thread holds one
+// y task of each step
+template <int N> constexpr bool inv_prefetch_fits() {
+    return (N / 2 + 1) * Geo<N>::N1 <= 1024 && (N / 2 + 1) * Geo<N>::N2 <= 1024;
+}
+
+// LEVEL 2: every variant; LEVEL 1: running lists with (3) or without (2) the prefetch
+template <int N, int TT, int LEVEL> void launch_inv_yz(const PipeArgs &a, hipStream_t s) {
+    const int nplanes = a.B * N;
+    const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
+    if constexpr (LEVEL < 2) {
+        if (a.inv_variant == 3 && inv_prefetch_fits<N>())
+            kp_inv_yz_top<N, TT, inv_prefetch_fits<N>(), true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp,
+                                                                                   a.t, a.nf3f, nplanes);
+        else
+            kp_inv_yz_top<N, TT, false, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f,
+                                                                  nplanes);
     } else switch (a.inv_variant) {
         case 1:
             kp_inv_yz_top<N, TT, true><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
@@ -818,10 +1153,12 @@ template <int N> constexpr int fwd_zy_chunk() {
     return 0;
 }
 
-// FULL: every kernel variant (the tuned sizes); otherwise the defaults only (correlation
-// variant 6, fused-inverse variant 3 up to nf = 128 where it fits, else 2), whatever the
-// context asks for.
-template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
+// LEVEL 2: every kernel variant (the sizes tuned most, 80, 96, 144, 256); LEVEL 1: the
+// variants worth tuning on every size (forward 0 and 2, correlation 5, 6 and 7, inverse 2
+// and 3).  A variant not built for the size runs as the nearest one built (correlation 6,
+// inverse 2).
+template <int N, int LEVEL> hipError_t run_pipe(const PipeArgs &a) {
+    constexpr bool FULL = LEVEL >= 2;
     static_assert(pipe_size_ok<N>(), "no specialised pipeline for this size");
     constexpr int T = Geo<N>::T;
     constexpr int L = 16, LX = 8;
@@ -829,15 +1166,24 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     if (ng != N / 2 || a.nd != N / 4) return hipErrorInvalidValue;  // the kernels assume nf = 2 ng, nd = ng / 2
     hipStream_t s = a.stream;
     constexpr int CZY = fwd_zy_chunk<N>();
-    bool fused_fwd = false;
-    if constexpr (CZY > 0) {
-        if (!FULL || a.fwd_variant == 0 || a.fwd_variant == 2) {  // fused z + y (2: prefetching)
+    bool fused_fwd = false, perm = false;
+    if constexpr (CZY == 0 && fwd_half_fits<N>()) {
+        if (a.fwd_variant != 1) {  // fused z + y over half planes; d_a2 in parity-split kz order
             const int nplanes = B * ng;
             const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;
-            if (FULL && a.fwd_variant == 2)
-                kp_fwd_zy<N, CZY, true><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+            kp_fwd_half<N><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+            if (a.ev) (void)hipEventRecord(a.ev[0], s);  // the z stage is fused into the y stage
+            fused_fwd = perm = true;
+        }
+    }
+    if constexpr (CZY > 0) {
+        if (!FULL || a.fwd_variant == 0 || a.fwd_variant == 2) {  // fused z + y, the next plane prefetched (2: not)
+            const int nplanes = B * ng;
+            const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;
+            if (a.fwd_variant == 2)
+                kp_fwd_zy<N, CZY, false><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
             else
-                kp_fwd_zy<N, CZY><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
+                kp_fwd_zy<N, CZY, true><<<grid, 1024, 0, s>>>(a.vox, a.a2, a.tw4, nplanes);
             if (a.ev) (void)hipEventRecord(a.ev[0], s);  // the z stage is fused into the y stage
             fused_fwd = true;
         }
@@ -852,8 +1198,14 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     }
     if (a.ev) (void)hipEventRecord(a.ev[1], s);
     if (a.spectrum_only) {
-        kp_x_corr<N, LX, 1, false, true, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
-                                                                                                  a.H, a.Hp, B);
+        if (perm) {
+            if constexpr (fwd_half_fits<N>())
+                kp_x_corr<N, LX, 1, false, true, true, true, false, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(
+                    a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+        } else {
+            kp_x_corr<N, LX, 1, false, true, true, true><<<dim3(B, a.Hp / LX, N), 2 * LX * T, 0, s>>>(a.a2, a.rec, a.cc,
+                                                                                                      a.tw4, a.H, a.Hp, B);
+        }
         return hipGetLastError();
     }
     // correlation-pass variant: one rotation per workgroup with the rotation index fastest in
@@ -864,9 +1216,26 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     // (3); receptor tile cached in registers for 4 rotations (0); re-read from L2 for each
     // of 4 rotations (1); one rotation per workgroup, tile index fastest (2); as 6 without
     // the padding columns (7)
-    if constexpr (!FULL) {
-        kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
-            a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+    constexpr int LC = 2 * LX, NCH = (N / 2 + 1) / LC, NFL = ((N / 2 + 1) % LC * N + LC - 1) / LC;
+    if (perm) {  // the parity-split d_a2: variant 6's tiling
+        if constexpr (fwd_half_fits<N>())
+            kp_x_corr<N, 2 * LX, 1, true, true, true, false, false, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+    } else if constexpr (!FULL) {
+        if (a.variant == 5) {
+            kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+        } else if (a.variant == 7 && NFL > 0) {
+            if constexpr (NCH > 0)
+                kp_x_corr<N, LC, 1, true, true, true><<<dim3(B, NCH, N), 2 * LC * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
+                                                                                           a.H, a.Hp, B);
+            if constexpr (NFL > 0)
+                kp_x_corr<N, LC, 1, true, true, true, false, true><<<dim3(B, NFL, 1), 2 * LC * T, 0, s>>>(
+                    a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+        } else {
+            kp_x_corr<N, 2 * LX, 1, true, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+        }
     } else switch (a.variant) {
         case 0:
             kp_x_corr<N, LX, 4, true><<<dim3(a.Hp / LX, N, (B + 3) / 4), 2 * LX * T, 0, s>>>(
@@ -891,7 +1260,6 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
                 a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
             break;
         case 7: {  // as 6 over the whole chunks of 16 columns, the remaining columns as a flat list
-            constexpr int LC = 2 * LX, NCH = (N / 2 + 1) / LC, NFL = ((N / 2 + 1) % LC * N + LC - 1) / LC;
             if constexpr (NCH > 0)
                 kp_x_corr<N, LC, 1, true, true, true><<<dim3(B, NCH, N), 2 * LC * T, 0, s>>>(a.a2, a.rec, a.cc, a.tw4,
                                                                                            a.H, a.Hp, B);
@@ -908,19 +1276,32 @@ template <int N, bool FULL> hipError_t run_pipe(const PipeArgs &a) {
     if (a.ev) (void)hipEventRecord(a.ev[2], s);
     if constexpr (fused_inverse<N>()) {
         if (a.ev) (void)hipEventRecord(a.ev[3], s);  // inverse y is fused into the next pass
-        if (a.t <= 1) launch_inv_yz<N, 1, FULL>(a, s);
-        else if (a.t <= 2) launch_inv_yz<N, 2, FULL>(a, s);
-        else if (a.t <= 4) launch_inv_yz<N, 4, FULL>(a, s);
-        else if (a.t <= 8) launch_inv_yz<N, 8, FULL>(a, s);
-        else launch_inv_yz<N, 16, FULL>(a, s);
+        if (a.t <= 1) launch_inv_yz<N, 1, LEVEL>(a, s);
+        else if (a.t <= 2) launch_inv_yz<N, 2, LEVEL>(a, s);
+        else if (a.t <= 4) launch_inv_yz<N, 4, LEVEL>(a, s);
+        else if (a.t <= 8) launch_inv_yz<N, 8, LEVEL>(a, s);
+        else launch_inv_yz<N, 16, LEVEL>(a, s);
     } else {
-        kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
-        if (a.ev) (void)hipEventRecord(a.ev[3], s);
-        if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
-        else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
-        else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
-        else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
-        else launch_inv_z<N, L, 16>(a, s);
+        bool half = false;
+        if constexpr (half_inverse_fits<N>()) half = !FULL || a.inv_variant != 0;
+        if (half) {  // the half-plane fused inverse (default where it fits; variant 0: two passes)
+            if constexpr (half_inverse_fits<N>()) {
+                if (a.ev) (void)hipEventRecord(a.ev[3], s);  // inverse y is fused into the next pass
+                if (a.t <= 1) launch_inv_half<N, 1>(a, s);
+                else if (a.t <= 2) launch_inv_half<N, 2>(a, s);
+                else if (a.t <= 4) launch_inv_half<N, 4>(a, s);
+                else if (a.t <= 8) launch_inv_half<N, 8>(a, s);
+                else launch_inv_half<N, 16>(a, s);
+            }
+        } else {
+            kp_inv_y<N, L><<<dim3(a.Hp / L, N, B), L * T, 0, s>>>(a.cc, a.tw4, a.Hp);
+            if (a.ev) (void)hipEventRecord(a.ev[3], s);
+            if (a.t <= 1) launch_inv_z<N, L, 1>(a, s);
+            else if (a.t <= 2) launch_inv_z<N, L, 2>(a, s);
+            else if (a.t <= 4) launch_inv_z<N, L, 4>(a, s);
+            else if (a.t <= 8) launch_inv_z<N, L, 8>(a, s);
+            else launch_inv_z<N, L, 16>(a, s);
+        }
     }
     if (a.ev) (void)hipEventRecord(a.ev[4], s);
     return hipGetLastError();

@@ -58,6 +58,70 @@ def dispatch_stats(trace_dir):
     return rows
 
 
+def busy(trace_dir, gap_ms=0.2, split_ms=50.0):
+    """GPU-busy share of a kernel trace: the union of the dispatches' [start, end] intervals
+    over the span from the first docking kernel to the last one (kernels named kp_ or k_),
+    and the idle gaps longer than gap_ms.  Gaps longer than split_ms (a process ending and
+    the next starting, e.g. a warm-up run before the timed one) split the trace into
+    segments; the longest segment is reported apart ("main"), with its gaps by length and
+    the kernels on either side of its longest gaps."""
+    iv = []
+    for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv"))):
+        k = short_name(r["Kernel_Name"])
+        if PIPELINE.match(k) or k.startswith("k_"):
+            iv.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), k.split("<")[0]))
+    if not iv:
+        return None
+    iv.sort()
+
+    def scan(seg):
+        covered, idle, gl = 0, 0, []
+        cs, ce, last = seg[0][0], seg[0][1], seg[0][2]
+        for s, e, k in seg[1:]:
+            if s > ce:
+                covered += ce - cs
+                idle += s - ce
+                gl.append((s - ce, last, k))
+                cs, ce = s, e
+            else:
+                ce = max(ce, e)
+            if e >= ce:
+                last = k
+        covered += ce - cs
+        return covered, idle, ce - seg[0][0], gl
+
+    covered, idle, span, gl = scan(iv)
+    out = {"span_ms": span / 1e6, "busy_ms": covered / 1e6, "busy_frac": covered / span, "idle_ms": idle / 1e6,
+           "gaps_over_%gms" % gap_ms: sum(g > gap_ms * 1e6 for g, _, _ in gl), "dispatches": len(iv)}
+    # segments split at the long gaps; the longest is the main run
+    segs, cur, end = [], [iv[0]], iv[0][1]
+    for x in iv[1:]:
+        if x[0] - end > split_ms * 1e6:
+            segs.append(cur)
+            cur = []
+        cur.append(x)
+        end = max(end, x[1])
+    segs.append(cur)
+    main = max(segs, key=lambda sg: max(e for _, e, _ in sg) - sg[0][0])
+    mc, mi, ms, mg = scan(main)
+    edges = [0.2, 0.5, 1, 2, 5, split_ms]
+    hist = {}
+    for lo, hi in zip(edges, edges[1:]):
+        sel = [g for g, _, _ in mg if lo * 1e6 < g <= hi * 1e6]
+        hist["%g-%g ms" % (lo, hi)] = {"gaps": len(sel), "ms": round(sum(sel) / 1e6, 2)}
+    pairs = collections.defaultdict(lambda: [0, 0])
+    for g, a, b in mg:
+        if g > gap_ms * 1e6:
+            pairs[a + " -> " + b][0] += 1
+            pairs[a + " -> " + b][1] += g
+    top = sorted(pairs.items(), key=lambda kv: -kv[1][1])[:8]
+    out["segments"] = len(segs)
+    out["main"] = {"span_ms": ms / 1e6, "busy_ms": mc / 1e6, "busy_frac": mc / ms, "idle_ms": mi / 1e6,
+                   "dispatches": len(main), "gaps_by_length": hist,
+                   "around_gaps": {k: {"gaps": v[0], "ms": round(v[1] / 1e6, 2)} for k, v in top}}
+    return out
+
+
 def counter_values(pmc_dir):
     """{counter: {kernel: [(grid, value per dispatch), ...]}} of one counter pass."""
     vals = collections.defaultdict(lambda: collections.defaultdict(list))
@@ -111,6 +175,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("trace")
     t.add_argument("dir")
+    t.add_argument("--busy", action="store_true", help="also the GPU-busy share of the docking span")
     p = sub.add_parser("pmc")
     p.add_argument("dirs", nargs="+")
     p.add_argument("--batch", type=int, required=True)
@@ -129,6 +194,9 @@ def main(argv=None):
                 continue
             print("| %s | %d | %d | %.1f | %.1f | %.2f |" % (r["kernel"], r["grid"], r["calls"], r["avg_us"],
                                                              r["median_us"], r["total_ms"]))
+        if a.busy:
+            b = busy(a.dir)
+            print("\nGPU busy over the docking span (union of the kernels' intervals): %s" % json.dumps(b))
         return 0
     s = pmc_summary(a.dirs, a.batch, a.copy_bytes)
     with open(a.json, "w") as fh:

@@ -1,6 +1,7 @@
 """GPU parity on the synthetic workloads other than the benchmark's: c0 (small, auto
 grid, t=1), c2 (pairs of uneven size and grid), c4 (256^3, t=3) on samples of their
-rotations against the CPU oracle, and c3 (refinement of the best c1 decoys)."""
+rotations against the CPU oracle, and c3 (refinement of the best c1 decoys: of a sample,
+and the best 2000 of the full 54000-rotation run)."""
 import numpy as np
 import pytest
 
@@ -93,4 +94,32 @@ def test_c3_refines_the_best_c1_decoys(wl):
     _say("c3 hard scores %s -> %s" % (hist[0][:4], hist[-1][:4]))
     assert np.allclose(hist[0], top["score"], rtol=1e-5, atol=1e-6)
     assert np.all(np.diff(hist, axis=0) >= 0)
+    ctx.close()
+
+
+def test_c3_refines_the_best_2000_of_the_full_c1_run(wl):
+    """The config as stated: all 54000 rotations of c1 docked (t = 3), the best 2000 decoys
+    refined on the hard score; no decoy's hard score drops, every starting hard score is the
+    docked score, and the refinement raises some of them."""
+    from megadock_amd import refine as F
+    rec_pdb, lig_pdb, cfg = W.make_pair("c1", wl)
+    prec, plig = M.Protein(rec_pdb), M.Protein(lig_pdb)
+    ng = cfg["nf"] // 2
+    ctx = M.Context(cfg["nf"], ng, plig.na, t=cfg["t"], max_batch=128)
+    ctx.set_receptor(*M.receptor_grids(prec, ng))
+    ctx.set_ligand(plig.xyz, plig.center, plig.radius, plig.charge, M.grid_coord(ng), 1.2)
+    rc = M.rotation_coefs(O.load_rotations(54000))
+    ctx.set_rotations(rc)
+    _say("c3: docking all 54000 c1 rotations")
+    ctx.dock_range(0, len(rc))
+    ctx.sync()
+    top = ctx.topn(len(rc), 2000)
+    assert len(top) == 2000 and np.all(np.diff(top["score"]) <= 0)
+    rot = rc[top["index"][:, 0]]
+    _say("c3: refining the best 2000 decoys")
+    R, t, hist = F.refine_hard(ctx, rot, top["index"][:, 1:].astype(np.float32), steps=3)
+    _say("c3: improved %d of 2000" % int(np.sum(hist[-1] > hist[0])))
+    assert np.allclose(hist[0], top["score"], rtol=1e-5, atol=1e-6)
+    assert np.all(np.diff(hist, axis=0) >= 0)
+    assert np.sum(hist[-1] > hist[0]) > 0
     ctx.close()

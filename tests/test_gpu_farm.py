@@ -1,8 +1,8 @@
 """GPU: the job-table farm (mdkh_run_table) docks every job of a table exactly as the
-single-pair program does (byte-identical .out, .detail and .csv files), with two contexts per
-FFT size reused across jobs (ligands of fewer atoms than the context was made for; one job
-set up and queued while the previous one runs), retries a failing job before it reports it,
-and keeps its host memory bounded on a long table."""
+single-pair program does (byte-identical .out, .detail and .csv files), with one context per
+FFT size reused across jobs (ligands of fewer atoms than the context was made for; a job of
+another size set up and queued while the previous one runs), retries a failing job before it
+reports it, and keeps its host memory bounded on a long table."""
 import os
 import re
 import subprocess
@@ -62,11 +62,10 @@ def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):
 
 def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
     """Ten FFT sizes, then two of the first ones again, then one size three times in a row:
-    the farm keeps at most eight contexts alive, frees the least recently used one (never
-    the one whose job is in flight) for each new size, makes a context again (with its
-    rotation table) when an evicted size comes back, and alternates between two contexts of
-    one size for consecutive jobs of that size.  Every job's .out equals the single-pair
-    program's."""
+    the farm keeps at most six contexts alive, frees the least recently used one (never the
+    one whose job is in flight) for each new size, makes a context again (with its rotation
+    table) when an evicted size comes back, and lets a job of the in-flight job's size wait
+    for it (they share the context).  Every job's .out equals the single-pair program's."""
     r = os.path.join(golden_dir, "1gcq_r.pdb")
     l = os.path.join(golden_dir, "1gcq_l.pdb")
     sizes = [80, 84, 88, 90, 96, 98, 100, 108, 110, 112, 80, 88, 96, 96, 96]

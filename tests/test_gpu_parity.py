@@ -358,6 +358,8 @@ def test_topn_device_merge_all_tied(gpu, gcq):
 
 
 @pytest.mark.parametrize("ng,env", [(48, {"MDK_XCORR_VARIANT": v}) for v in "01234567"] +
+                         [(50, {"MDK_XCORR_VARIANT": v}) for v in "57"] + [(50, {"MDK_INV_VARIANT": "2"}),
+                                                                           (50, {"MDK_FWD_VARIANT": "2"})] +
                          [(48, {"MDK_INV_VARIANT": v}) for v in "123"] +
                          [(48, {"MDK_STREAMS": "1"}), (48, {"MDK_FWD_VARIANT": "1"}), (48, {"MDK_FWD_VARIANT": "2"})] +
                          [(72, {"MDK_XCORR_VARIANT": v}) for v in "4567"] + [(72, {"MDK_FWD_VARIANT": v}) for v in "12"] +
@@ -457,9 +459,9 @@ def test_autotune_keeps_results(gpu, gcq):
     ctx.dock_range(0, 256)
     ctx.sync()
     before = ctx.results(0, 256)
-    xv, iv = ctx.autotune(0, 128)
-    _say("autotune chose correlation variant %d, inverse variant %d" % (xv, iv))
-    assert 3 <= xv <= 7 and 0 <= iv <= 3
+    xv, iv, fv = ctx.autotune(0, 128)
+    _say("autotune chose correlation variant %d, inverse variant %d, forward variant %d" % (xv, iv, fv))
+    assert 3 <= xv <= 7 and 0 <= iv <= 3 and fv in (0, 1, 2)
     ctx.dock_range(0, 256)
     ctx.sync()
     assert np.array_equal(before.view(np.uint8), ctx.results(0, 256).view(np.uint8))
@@ -513,3 +515,50 @@ def test_graph_replays_match_eager(gpu, gcq):
     g.sync()
     assert np.array_equal(g.results(0, 300).view(np.uint8), ref2.view(np.uint8))
     g.close()
+
+
+_NF256_REF = {}
+
+
+@pytest.mark.parametrize("inv,fwd", [("0", "1"), ("2", "0"), ("2", "1"), ("0", "0")])
+def test_nf256_inverse_forms_vs_oracle(gpu, synth, inv, fwd):
+    """At nf = 256 a plane does not fit one workgroup's LDS.  The default fused forward pass
+    takes each z-transformed plane as its even-kz and odd-kz halves and splits the y
+    transform by ky parity (MDK_FWD_VARIANT 0), variant 1 runs the z and y passes apart; the
+    default fused inverse takes each plane as its even and odd rows (a radix-2 split of the
+    y transform, MDK_INV_VARIANT 1-3), variant 0 runs the in-place y pass and the z pass
+    apart.  Each pairing against the oracle, per rotation, for t = 1 and t = 3 lists."""
+    rec, lig = synth
+    ng, nf = 128, 256
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    remb = O.embed(ng, nf, gr, gi)
+    ang = O.load_rotations(3600)
+    ids = np.array([5, 1717, 3333], dtype=np.int32)
+    for t in (1, 3):
+        env = {"MDK_INV_VARIANT": inv, "MDK_FWD_VARIANT": fwd}
+        got = _with_env(env, lambda: _dock_ids(lig, ng, t, gr, gi, ang, ids))
+        _say("nf 256 inverse variant %s, forward variant %s, t %d: %d rotations" % (inv, fwd, t, len(ids)))
+        if t not in _NF256_REF:
+            rr, ri = O.receptor_fft(ng, nf, gr, gi)
+            _NF256_REF[t] = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, t, threads=16)
+        ref = _NF256_REF[t]
+        for q in range(len(ids) * t):
+            g, r = got[q], ref[q]
+            assert g["index"][0] == r["index"][0]
+            assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
+            if not np.array_equal(g["index"], r["index"]):
+                a = int(ids[q // t])
+                xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+                lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+                s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+                ig = (g["index"][1] * nf + g["index"][2]) * nf + g["index"][3]
+                ir = (r["index"][1] * nf + r["index"][2]) * nf + r["index"][3]
+                assert _near_tie_ok(s64, ig, ir, 2e-6), (q, g, r)
+
+
+def _dock_ids(lig, ng, t, gr, gi, ang, ids):
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=4)
+    ctx.set_receptor(gr, gi)
+    out = ctx.dock(M.rotation_coefs(ang[ids]), ids)
+    ctx.close()
+    return out

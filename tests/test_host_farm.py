@@ -66,7 +66,7 @@ def test_specialised_size_lists_match_the_build():
     src = open(os.path.join(ROOT, "megadock_amd", "csrc", "mdk_pipe.hip")).read()
     listed = [int(x) for x in re.findall(r"X\((\d+)\)", src)]
     assert listed == M.FURTHER_NF
-    assert all("case %d: return run_pipe<%d, true>" % (n, n) in src for n in M.TUNED_NF)
+    assert all("case %d: return run_pipe<%d, 2>" % (n, n) in src for n in M.TUNED_NF)
 
 
 def test_command_line_parses_afresh_after_a_trailing_flag(tmp_path):
