@@ -20,6 +20,7 @@ Also reported on the same JSON line:
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -61,10 +62,12 @@ def kernel_table(stage_ms, nfull, nf, batch):
     empty = lambda k: ms[k] < 0.02 * max(ms.values())
     if empty("fwd_y"):
         passes["fwd_z"] += passes.pop("fwd_y")
-        names["fwd_z"] = "kp_fwd_zy (forward z + y)"
+        names["fwd_z"] = ("kp_fwd_half (forward z + y over half planes)" if nf == 256
+                          else "kp_fwd_zy (forward z + y)")
     if empty("inv_y"):
         passes["inv_z_top"] += passes.pop("inv_y")
-        names["inv_z_top"] = "kp_inv_yz_top (inverse y + z + top-t)"
+        names["inv_z_top"] = ("kp_inv_yz_half (inverse y + z + top-t over half planes)" if nf == 256
+                              else "kp_inv_yz_top (inverse y + z + top-t)")
     rows = {}
     for k in passes:
         share = passes[k] / 6.0
@@ -160,9 +163,25 @@ def bench_c2(args, M, workloads):
                    "pairs_on_specialised_kernels": sum(1 for n in nfs if n in spec)},
         "roofline": roof,
         "farm_seconds": round(farm_s, 3), "pairs_per_s": round(len(pairs) / farm_s, 2),
+        "farm_log": farm_log_summary(open(flog).read()),
         "one_after_another": seq,
     }), flush=True)
     return 0
+
+
+def farm_log_summary(text):
+    """The farm log's last lines: the most prepared jobs waiting at once (and the limit), how
+    often the GPU drained between jobs and the contexts made and freed."""
+    out = {}
+    m = re.search(r"at most (\d+) \(limit (\d+)\)", text)
+    if m:
+        out["prepared_waiting_max"], out["prepared_waiting_limit"] = int(m.group(1)), int(m.group(2))
+    m = re.search(r"drained between jobs: (\d+) with no job prepared, (\d+) for a job of the running job's FFT size; "
+                  r"contexts made (\d+), freed (\d+)", text)
+    if m:
+        out.update({"drained_no_job_prepared": int(m.group(1)), "drained_same_fft_size": int(m.group(2)),
+                    "contexts_made": int(m.group(3)), "contexts_freed": int(m.group(4))})
+    return out
 
 
 def bench_c3(args, M, workloads):

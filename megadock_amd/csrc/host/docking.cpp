@@ -29,7 +29,10 @@ const float PI_F = (float)(4.0 * std::atan(1.0));  // src/constant.h:24
 
 }  // namespace
 
-int auto_batch(int nf, int ng, int t) {
+namespace {
+// device bytes of one rotation's batch buffers (d_a1, d_a2, d_cc, d_cand; for t > 16 the
+// score map and its select keys)
+double batch_bytes_per_rotation(int nf, int ng, int t) {
     const double H = nf / 2 + 1;
     double per_rot = 8.0 * ((double)ng * ng * nf + (double)ng * nf * nf + (double)nf * nf * (H + 16) +
                             (double)nf * nf * std::min(t, 16));
@@ -38,6 +41,20 @@ int auto_batch(int nf, int ng, int t) {
         while (P < t) P *= 2;
         per_rot += 4.0 * nf * nf * nf + 8.0 * P;
     }
+    return per_rot;
+}
+}  // namespace
+
+double context_bytes(int nf, int ng, int t, int batch, int na) {
+    // two batch slots (with their ligand grids, charge grids and atom indices), the receptor
+    // spectrum and grids, and room for the rest
+    const double ng3 = (double)ng * ng * ng;
+    return 2.0 * batch * (batch_bytes_per_rotation(nf, ng, t) + 12.0 * ng3 + 4.0 * na) + 8.0 * nf * nf * nf +
+           12.0 * ng3 + 64.0 * 1024 * 1024;
+}
+
+int auto_batch(int nf, int ng, int t) {
+    const double per_rot = batch_bytes_per_rotation(nf, ng, t);
     // 8 GiB of batch buffers, at most 256 rotations a batch up to nf = 128 (184k against 175k
     // rotations/s at 96^3) and 128 up to nf = 192 (144^3: 61.8k for either, steadier at 128);
     // above, 36 GiB and at most 192 (256^3: 6.85k at 192 against 6.56k at 48 rotations)

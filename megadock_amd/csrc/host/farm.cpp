@@ -58,6 +58,7 @@ double now_s() {
 struct FarmOptions {
     std::string table, log = "./master.log";
     int out_option = 0, retries = 0, prep_threads = 0;
+    double context_gib = 160.0;  // -gm: device memory for the live contexts (GiB)
     bool dry = false;
     std::vector<std::string> app;  // options passed to every job
 };
@@ -90,6 +91,13 @@ int parse_options(int argc, char **argv, FarmOptions &o, std::string &err) {
         } else if (a == "-pj") {
             if (!value(v)) return 1;
             o.prep_threads = std::atoi(v.c_str());
+        } else if (a == "-gm") {  // not in the reference: this farm keeps a context per FFT size
+            if (!value(v)) return 1;
+            o.context_gib = std::atof(v.c_str());
+            if (!(o.context_gib > 0)) {
+                err = "option -gm needs a positive number of GiB";
+                return 1;
+            }
         } else if (a == "-dry") {
             o.dry = true;
         } else {
@@ -317,14 +325,19 @@ struct Slot {
     int na = 0, t = 0, batch = 0;
     int rot_set = -1;  // rotation set loaded into the context
     long last_use = 0;
+    double bytes = 0;  // device bytes (context_bytes)
 };
 
-// One context per FFT size, at most kMaxLive at a time (each holds up to two batches of
-// about 8 GiB of buffers): a table with more sizes than that frees the least recently used
-// context (never the busy one, whose job is in flight) before it creates another.
+// One context per FFT size, kept alive while their device buffers fit a budget (-gm, 160
+// GiB of the GPU's 288 GB by default; context_bytes estimates each): a table
+// with more sizes than fit frees the least recently used contexts (never the busy one,
+// whose job is in flight) before it creates another.  Freeing a context waits for the
+// whole device, so a context is made with room for ligands of up to kMinAtoms atoms and
+// is not remade for a ligand that fits it.
 class Contexts {
 public:
-    static constexpr int kMaxLive = 6;
+    static constexpr int kMinAtoms = 16384;
+    explicit Contexts(double budget_bytes) : budget_(budget_bytes) {}
     ~Contexts() {
         for (auto &kv : slots_)
             if (kv.second.ctx) mdk_destroy(kv.second.ctx);
@@ -333,27 +346,32 @@ public:
     // context and the rotation set loaded into it); `busy` (the in-flight job's context,
     // another size) is never freed
     Slot *get(const Job &j, mdk_ctx *busy, std::string &err) {
-        const int nf = 2 * j.ng, na = j.lig->num_atoms(), t = j.prm.num_sort;
+        const int nf = 2 * j.ng, t = j.prm.num_sort;
         int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
         batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
+        const int na = std::max(j.lig->num_atoms(), std::min(kMinAtoms, 65535));
         Slot *s = &slots_[nf];
         s->last_use = ++clock_;
-        if (s->ctx && (s->na < na || s->t != t || s->batch < batch)) {
+        if (s->ctx && (s->na < j.lig->num_atoms() || s->t != t || s->batch < batch)) {
             mdk_destroy(s->ctx);
+            destroyed++;
             *s = Slot();
         }
         if (!s->ctx) {
-            free_others(kMaxLive - 1, nf, busy);
+            const double need = context_bytes(nf, j.ng, t, batch, na);
+            free_to(budget_ - need, nf, busy);
             int rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);
-            if (rc && free_others(0, nf, busy) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);  // out of memory?
+            if (rc && free_to(0, nf, busy) > 0) rc = mdk_create(0, nf, j.ng, na, t, batch, &s->ctx);  // out of memory?
             if (rc) {
                 err = std::string("mdk_create: ") + mdk_last_error();
                 *s = Slot();
                 return nullptr;
             }
+            created++;
             s->na = na;
             s->t = t;
             s->batch = batch;
+            s->bytes = need;
             s->last_use = clock_;
         }
         return s;
@@ -362,28 +380,31 @@ public:
         auto it = slots_.find(nf);
         return it != slots_.end() && it->second.ctx != nullptr;
     }
+    int created = 0, destroyed = 0;
 
 private:
     // destroy the least recently used contexts other than size `keep_nf` and `busy` until
-    // at most `keep` of them are left; returns how many were destroyed
-    int free_others(int keep, int keep_nf, mdk_ctx *busy) {
+    // the others' bytes are at most `room`; returns how many were destroyed
+    int free_to(double room, int keep_nf, mdk_ctx *busy) {
         int freed = 0;
         for (;;) {
-            int n = 0;
+            double live = 0;
             Slot *lru = nullptr;
             for (auto &kv : slots_)
-                if (kv.first != keep_nf && kv.second.ctx && kv.second.ctx != busy) {
-                    n++;
-                    if (!lru || kv.second.last_use < lru->last_use) lru = &kv.second;
+                if (kv.first != keep_nf && kv.second.ctx) {
+                    live += kv.second.bytes;
+                    if (kv.second.ctx != busy && (!lru || kv.second.last_use < lru->last_use)) lru = &kv.second;
                 }
-            if (n <= keep || !lru) return freed;
+            if (live <= room || !lru) return freed;
             mdk_destroy(lru->ctx);
+            destroyed++;
             *lru = Slot();
             freed++;
         }
     }
     std::map<int, Slot> slots_;
     long clock_ = 0;
+    double budget_;
 };
 
 // The device part of a job in two calls: issue() uploads the job and queues its batches
@@ -615,8 +636,9 @@ int run_table(int argc, char **argv, std::string &err) {
             std::lock_guard<std::mutex> g(ready_mu);
             if (--prep_alive == 0) ready.close();
         });
+    int drain_empty = 0, drain_same = 0, ctx_made = 0, ctx_freed = 0;  // the GPU's waits between jobs
     {
-        Contexts cs;
+        Contexts cs(o.context_gib * 1024 * 1024 * 1024);
         // inflight: issued, its merge not yet waited for; written: merged, its .out file
         // still to write
         std::unique_ptr<Job> j, inflight, written;
@@ -656,12 +678,16 @@ int run_table(int argc, char **argv, std::string &err) {
         auto live = [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); };
         for (;;) {
             if (!ready.try_pop_prefer(j, live_other)) {
+                if (inflight) drain_empty++;
                 complete_inflight();  // nothing prepared: finish what is queued before waiting
                 write_pending();
                 if (!ready.pop_prefer(j, live)) break;
             }
             // a job of the in-flight job's size shares its context: that job completes first
-            if (inflight && inflight->ng == j->ng) complete_inflight();
+            if (inflight && inflight->ng == j->ng) {
+                drain_same++;
+                complete_inflight();
+            }
             bool ok = false;
             try {
                 ok = issue(*j, cs, inflight ? inflight->ctx : nullptr) == 0;  // queued behind the job in flight
@@ -675,6 +701,8 @@ int run_table(int argc, char **argv, std::string &err) {
         }
         complete_inflight();
         write_pending();
+        ctx_made = cs.created;
+        ctx_freed = cs.destroyed;
     }
     for (auto &w : workers) w.join();
 
@@ -695,6 +723,8 @@ int run_table(int argc, char **argv, std::string &err) {
     }
     const double el = now_s() - t_start;
     log << "\nPrepared jobs waiting for the GPU: at most " << ready.high_water() << " (limit " << 2 * nprep << ")";
+    log << "\nGPU drained between jobs: " << drain_empty << " with no job prepared, " << drain_same
+        << " for a job of the running job's FFT size; contexts made " << ctx_made << ", freed " << ctx_freed;
     log << "\nJobs done = " << njobs - failed << " / " << njobs << "\nElapsed time  = " << el << " sec.\n"
         << "Rotations docked = " << rotations << " (" << (el > 0 ? rotations / el : 0.0) << " per second)\n";
     if (failed) {

@@ -190,8 +190,13 @@ int write_decoys(const Parameter &prm, const Protein &rec, const Protein &lig, i
 int write_detail(const Parameter &prm, const Receptor &rec, const Ligand &lig, int num_grid, const SortScore *top,
                  mdk_ctx *ctx, std::string &err);
 
-// Rotations per device batch for an FFT size: 8 GiB of batch buffers, at most 128.
+// Rotations per device batch for an FFT size: 8 GiB of batch buffers (36 GiB above nf =
+// 192), at most 256 up to nf = 128, 128 up to nf = 192 and 192 above.
 int auto_batch(int nf, int ng, int t);
+
+// Device bytes of a context of that size and batch (mdk_create's buffers), an estimate
+// for keeping several contexts alive within a memory budget.
+double context_bytes(int nf, int ng, int t, int batch, int na);
 
 // The job-table farm (src/mpidp.cpp), in process: see farm.cpp.
 int run_table(int argc, char **argv, std::string &err);

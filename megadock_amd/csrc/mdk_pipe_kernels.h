@@ -111,9 +111,12 @@ __global__ void __launch_bounds__(L *Geo<N>::T) kp_fwd_y(const float2 *__restric
 //   y-FFT of the nf columns kz in chunks of C, the ng non-zero rows at y + nd read from A,
 //   the outputs written straight to a2[b][x][ky][kz].
 // It replaces kp_fwd_z + kp_fwd_y and never writes a1.
+// Where ng rows of T threads exceed the workgroup (WIDE: nf = 150 - 176), step 1 of the z
+// transform takes N2 threads a row and step 2 two (row, k1) tasks a thread.
+template <int N> constexpr bool fwd_zy_wide() { return N / 2 * Geo<N>::T > 1024; }
 template <int N, int C> constexpr bool fwd_zy_fits() {
     constexpr int NG = N / 2, T = Geo<N>::T;
-    return NG * T <= 1024 && C * T <= 1024 && N % C == 0 &&
+    return NG * Geo<N>::N2 <= 1024 && NG * Geo<N>::N1 <= 2048 && C * T <= 1024 && N % C == 0 &&
            (size_t)N * (NG + 1) * 8 + (size_t)N * (C + 1) * 8 + (size_t)N * 8 <= 150 * 1024;
 }
 
@@ -124,7 +127,9 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
                                                    int nplanes) {
     constexpr int NG = N / 2, ND = N / 4;
     constexpr int N1 = Geo<N>::N1, N2 = Geo<N>::N2, T = Geo<N>::T;
-    constexpr int LA = NG + 1, LE = C + 1;
+    // A's row stride: odd, so that the y step's lanes (one column kz each, stride LA) spread
+    // over the LDS banks
+    constexpr int LA = NG | 1, LE = C + 1;
     static_assert(fwd_zy_fits<N, C>(), "fused forward pass does not fit");
     // one spare row after each group of N2 in the z exchange, where it still fits
     constexpr int PZ = (size_t)(N + N1) * LA * 8 + (size_t)N * LE * 8 + (size_t)N * 8 <= 150 * 1024 ? 1 : 0;
@@ -136,7 +141,9 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
     for (int q = threadIdx.x; q < N; q += 1024) tw[q] = twg[q];
     // the z-row task of this thread: row y of a plane, step-1 index u (u fastest, so that a
     // row's voxels sit in neighbouring lanes)
-    const int zy = (int)threadIdx.x / T, zu = (int)threadIdx.x - zy * T;
+    constexpr bool WIDE = fwd_zy_wide<N>();
+    constexpr int ZT = WIDE ? N2 : T;  // threads a row in step 1
+    const int zy = (int)threadIdx.x / ZT, zu = (int)threadIdx.x - zy * ZT;
     const bool zact = zy < NG && zu < N2;
     // a row's raw voxel data: its mask words and the charges of this thread's voxels
     uint32_t rd[RW], ro[RW];
@@ -173,7 +180,7 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
         // ---- z-FFT of row y: thread (y, u), the row's voxel values from the mask bits and
         // the charges into the step-1 registers; each non-zero charge read is cleared
         {
-            const int y = tid / T, u = tid - y * T;
+            const int y = tid / ZT, u = tid - y * ZT;
             const bool zrow = y < NG;
             if (zrow && u < N2) {
                 float *cr = vg.chg + (size_t)b * vg.ng3 + (size_t)(x * NG + y) * NG;
@@ -196,14 +203,32 @@ __global__ void __launch_bounds__(1024) kp_fwd_zy(VoxGrid vg, float2 *__restrict
                 rf::step1_g<N, false, PZ>(v, A, tw, 1, LA, 1, y, u);
             }
             __syncthreads();
-            float2 w[N2] = {};
-            if (zrow && u < N1) rf::step2_g<N, false, PZ>(w, A, LA, 1, y, u);
-            __syncthreads();
-            if (zrow && u < N1)
-                sfor<0, N2>([&](auto kc) {
-                    constexpr int k2 = decltype(kc)::value;
-                    A[(u + N1 * k2) * LA + y] = w[k2];
-                });
+            if constexpr (!WIDE) {
+                float2 w[N2] = {};
+                if (zrow && u < N1) rf::step2_g<N, false, PZ>(w, A, LA, 1, y, u);
+                __syncthreads();
+                if (zrow && u < N1)
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        A[(u + N1 * k2) * LA + y] = w[k2];
+                    });
+            } else {  // tasks (row, k1) number tid and tid + 1024
+                const int ya = tid / N1, ua = tid - ya * N1, yb = (tid + 1024) / N1, ub = tid + 1024 - yb * N1;
+                float2 wa[N2] = {}, wb[N2] = {};
+                if (ya < NG) rf::step2_g<N, false, PZ>(wa, A, LA, 1, ya, ua);
+                if (yb < NG) rf::step2_g<N, false, PZ>(wb, A, LA, 1, yb, ub);
+                __syncthreads();
+                if (ya < NG)
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        A[(ua + N1 * k2) * LA + ya] = wa[k2];
+                    });
+                if (yb < NG)
+                    sfor<0, N2>([&](auto kc) {
+                        constexpr int k2 = decltype(kc)::value;
+                        A[(ub + N1 * k2) * LA + yb] = wb[k2];
+                    });
+            }
         }
         __syncthreads();
         // ---- y-FFT of the columns, C at a time: thread (column, u), column fastest
@@ -954,6 +979,18 @@ __global__ void __launch_bounds__(1024)
 // same steps as kp_inv_yz_top (complex-to-real pre-step and M-point z-IFFT per row, running
 // top-t over a workgroup's contiguous planes), and the plane is read from HBM twice instead
 // of being transformed in place there (kp_inv_y) and read again (kp_inv_z_top).
+// The columns kz = 0 and kz = nf/2 are real after the y-IFFT (each z row of the score is
+// real, so its spectrum is Hermitian and both of these entries are real), so they go
+// through the y-IFFT together as one column, X(0) + i X(nf/2), and the pre-step takes them
+// apart as the real and imaginary parts: M columns instead of M + 1, whole rounds.
+// the input of row y = 2m + h's M-point y-IFFT at k from X[k] = a and X[k + M] = c;
+// w = tw[k] = conj(w^k)
+__device__ __forceinline__ float2 fold_half(float2 a, float2 c, int h, float2 w) {
+    if (h == 0) return make_float2(a.x + c.x, a.y + c.y);
+    const float2 d = make_float2(a.x - c.x, a.y - c.y);
+    return make_float2(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+}
+
 template <int N> constexpr bool half_inverse_fits() {
     constexpr int M = N / 2;
     return N % 4 == 0 && (size_t)M * plane_stride(N) * 8 + (size_t)N * 8 + 16 * 16 * 8 <= 150 * 1024 &&
@@ -965,7 +1002,7 @@ __global__ void __launch_bounds__(1024)
     kp_inv_yz_half(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                    const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
     constexpr int NTH = 1024, NW = NTH / 64;
-    constexpr int M = N / 2, H = M + 1;
+    constexpr int M = N / 2;
     constexpr int LSP = plane_stride(N);  // row stride of the half-plane (odd)
     constexpr int N1 = Geo<M>::N1, N2 = Geo<M>::N2;  // both transforms are M-point
     // the y transform runs in rounds of whole columns, one task per thread: step 1 over
@@ -973,7 +1010,8 @@ __global__ void __launch_bounds__(1024)
     // columns are read, then written back in place, so no thread holds two tasks' outputs);
     // the z transform over RS = NTH / N1 rows a round, its outputs pushed round by round
     constexpr int CS1 = NTH / N2, CS2 = NTH / N1, RS = NTH / N1;
-    constexpr int R1 = (H + CS1 - 1) / CS1, R2 = (H + CS2 - 1) / CS2, RZ = M / RS;
+    constexpr int HC = M;  // y columns: kz = 1 .. M - 1 and the packed pair kz = 0, M
+    constexpr int R1 = (HC + CS1 - 1) / CS1, R2 = (HC + CS2 - 1) / CS2, RZ = M / RS;
     static_assert(half_inverse_fits<N>(), "half-plane inverse does not fit");
     static_assert(NTH % N1 == 0 && NTH % N2 == 0 && M % RS == 0 && M * N2 <= NTH, "whole columns and rows a round");
     __shared__ float2 P[M * LSP];
@@ -1002,19 +1040,23 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll 1
             for (int r = 0; r < R1; r++) {  // uniform
                 const int yl = r * CS1 + tid % CS1, yu = tid / CS1;
-                if (yl < H) {
+                if (yl < HC) {
                     float2 v[N1];
                     sfor<0, N1>([&](auto jc) {
                         constexpr int j1 = decltype(jc)::value;
                         const int k = yu + N2 * j1;
                         const float2 a = src[(unsigned)(k * Hp + yl)], c = src[(unsigned)((k + M) * Hp + yl)];
-                        if (h == 0) {
-                            v[j1] = make_float2(a.x + c.x, a.y + c.y);
-                        } else {
-                            const float2 d = make_float2(a.x - c.x, a.y - c.y), w = tw[k];  // w^k = conj(tw[k])
-                            v[j1] = make_float2(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
-                        }
+                        v[j1] = fold_half(a, c, h, tw[k]);
                     });
+                    if (yl == 0) {  // column 0 carries X(0) + i X(M): add i times the kz = M fold
+                        asm volatile("" ::: "memory");  // after the main loads: no more of them in flight at once
+                        sfor<0, N1>([&](auto jc) {
+                            constexpr int j1 = decltype(jc)::value;
+                            const int k = yu + N2 * j1;
+                            const float2 q = fold_half(src[(unsigned)(k * Hp + M)], src[(unsigned)((k + M) * Hp + M)], h, tw[k]);
+                            v[j1] = make_float2(v[j1].x - q.y, v[j1].y + q.x);
+                        });
+                    }
                     rf::step1_g<M, true>(v, P, tw, 2, LSP, 1, yl, yu);
                 }
             }
@@ -1023,9 +1065,9 @@ __global__ void __launch_bounds__(1024)
             for (int r = 0; r < R2; r++) {  // uniform
                 const int yl = r * CS2 + tid % CS2, yu = tid / CS2;
                 float2 y[N2];
-                if (yl < H) rf::step2_g<M, true>(y, P, LSP, 1, yl, yu);
+                if (yl < HC) rf::step2_g<M, true>(y, P, LSP, 1, yl, yu);
                 __syncthreads();  // the round's columns are read before they are written back
-                if (yl < H)
+                if (yl < HC)
                     sfor<0, N2>([&](auto kc) {
                         constexpr int k2 = decltype(kc)::value;
                         P[(yu + N1 * k2) * LSP + yl] = y[k2];
@@ -1041,7 +1083,11 @@ __global__ void __launch_bounds__(1024)
                     sfor<0, N1>([&](auto jc) {
                         constexpr int j1 = decltype(jc)::value;
                         const int k = u + N2 * j1;
-                        const float2 a = P[line * LSP + k], c = P[line * LSP + M - k], w = tw[k];
+                        float2 a = P[line * LSP + k], c = P[line * LSP + (M - k) % M], w = tw[k];
+                        if (j1 == 0 && u == 0) {  // k = 0: X(0) and X(M) from the packed column
+                            c = make_float2(a.y, 0.0f);
+                            a = make_float2(a.x, 0.0f);
+                        }
                         const float2 s1 = make_float2(a.x + c.x, a.y - c.y);
                         const float2 d1 = make_float2(a.x - c.x, a.y + c.y);
                         const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
@@ -1146,7 +1192,7 @@ template <int N, int TT, int LEVEL> void launch_inv_yz(const PipeArgs &a, hipStr
 // the workgroup and LDS, or 0 where none does (then z and y run as two passes)
 template <int N> constexpr int fwd_zy_chunk() {
     constexpr int NG = N / 2, T = Geo<N>::T;
-    if (NG * T > 1024) return 0;
+    if (NG * Geo<N>::N2 > 1024 || NG * Geo<N>::N1 > 2048) return 0;
     for (int c = N; c >= 8; c--)
         if (N % c == 0 && c * T <= 1024 && (size_t)N * (NG + 1) * 8 + (size_t)N * (c + 1) * 8 + (size_t)N * 8 <= 150 * 1024)
             return c;

@@ -61,11 +61,12 @@ def test_farm_retries_then_reports_a_failing_job(tmp_path, golden_dir):
 
 
 def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
-    """Ten FFT sizes, then two of the first ones again, then one size three times in a row:
-    the farm keeps at most six contexts alive, frees the least recently used one (never the
-    one whose job is in flight) for each new size, makes a context again (with its rotation
-    table) when an evicted size comes back, and lets a job of the in-flight job's size wait
-    for it (they share the context).  Every job's .out equals the single-pair program's."""
+    """Ten FFT sizes, then two of the first ones again, then one size three times in a row,
+    with room for about three contexts (-gm 2 GiB): the farm frees the least recently used
+    context (never the one whose job is in flight) for each new size, makes a context again
+    (with its rotation table) when an evicted size comes back, and lets a job of the
+    in-flight job's size wait for it (they share the context).  Every job's .out equals the
+    single-pair program's."""
     r = os.path.join(golden_dir, "1gcq_r.pdb")
     l = os.path.join(golden_dir, "1gcq_l.pdb")
     sizes = [80, 84, 88, 90, 96, 98, 100, 108, 110, 112, 80, 88, 96, 96, 96]
@@ -74,9 +75,11 @@ def test_farm_with_more_fft_sizes_than_live_contexts(tmp_path, golden_dir):
     tb.write_text("TITLE=sizes\nPARAM=-R $1 -L $2 -o $3 -F $4 -r 24 -N 20 -W 8\n" + "\n".join(lines) + "\n")
     log = tmp_path / "master.log"
     print("farm of %d jobs over %d sizes" % (len(sizes), len(set(sizes))), flush=True)
-    M.run_table(["-tb", str(tb), "-lg", str(log), "-pj", "2"])
+    M.run_table(["-tb", str(tb), "-lg", str(log), "-pj", "2", "-gm", "2"])
     rows = _rows(log)
     assert all(rows["%05d" % (k + 1)][2] == "done" for k in range(len(sizes)))
+    made = re.search(r"contexts made (\d+), freed (\d+)", log.read_text())
+    assert made and int(made.group(1)) > 10 and int(made.group(2)) > 0, made
     for k, nf in enumerate(sizes):
         single = tmp_path / ("single_%d.out" % k)
         M.run(["-R", r, "-L", l, "-o", str(single), "-F", str(nf), "-r", "24", "-N", "20", "-W", "8"])
