@@ -60,7 +60,9 @@ int mdk_create(int device, int nf, int ng, int na_lig, int t, int max_batch, mdk
 int mdk_destroy(mdk_ctx *ctx);
 
 /* Receptor grids (ng^3 floats each, row-major x slowest): embedded in the nf^3 grid
- * and transformed on the device (FFTProcess::receptor_fft, src/fft_process.cpp:212-303). */
+ * and transformed on the device (FFTProcess::receptor_fft, src/fft_process.cpp:212-303).
+ * Returns once the grids are copied to a staging buffer; the upload and the transform
+ * are queued behind the context's queued batches, which see the previous receptor. */
 int mdk_set_receptor(mdk_ctx *ctx, const float *grid_r, const float *grid_i);
 
 /* Receptor spectrum given directly, SoA nf^3 floats each (the _FFT_rec_r/_FFT_rec_i
@@ -103,7 +105,8 @@ int mdk_set_ligand(mdk_ctx *ctx, const float *xyz, const float *center, const fl
 /* As mdk_set_ligand for a ligand of na atoms, 1 <= na <= the na_lig the context was
  * created with, so that one context serves the pairs of a job table that share an FFT
  * size.  No counterpart in the reference, which builds its FFTProcess per pair (the
- * table farm of src/mpidp.cpp runs one process per job). */
+ * table farm of src/mpidp.cpp runs one process per job).  Both ligand calls return once
+ * the arrays are staged; the upload is queued behind the context's queued batches. */
 int mdk_set_ligand_atoms(mdk_ctx *ctx, int na, const float *xyz, const float *center, const float *radius,
                          const float *charge, const float *grid_coord, float gw, float core_delta);
 
@@ -140,6 +143,14 @@ int mdk_get_results(mdk_ctx *ctx, int first, int count, mdk_sortscore *out);
  * by score descending, ties by rotation then rank ascending.  Replaces
  * FFTProcess::sort_index (src/fft_process.cpp:469-479), whose tie order is unspecified. */
 int mdk_topn(mdk_ctx *ctx, int nrot, int n, mdk_sortscore *out);
+
+/* mdk_topn in two calls: mdk_topn_start queues the merge behind the batches queued so far
+ * and returns; mdk_topn_finish waits for it and writes the best n to out.  Between them
+ * the caller may set up and queue other contexts' work, so the merge runs right after the
+ * context's last batch instead of behind work queued later.  One merge per context may be
+ * started at a time.  Extends the same reference step (src/fft_process.cpp:469-479). */
+int mdk_topn_start(mdk_ctx *ctx, int nrot, int n);
+int mdk_topn_finish(mdk_ctx *ctx, mdk_sortscore *out);
 
 /* As mdk_topn, allocating nothing: no hipMalloc, hipFree or allocating library call on
  * any call.  No counterpart in the reference; it is for callers that cannot allocate

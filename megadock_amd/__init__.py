@@ -24,6 +24,7 @@ ABI_SYMBOLS = [
     "mdk_score_map", "mdk_fft1d", "mdk_copy_probe", "mdk_score_spectral", "mdk_score_companion",
     "mdk_autotune", "mdk_set_ligand_atoms", "mdk_score_detail", "mdk_set_receptor_atoms", "mdk_get_receptor_grids",
     "mdk_decoy_coords", "mdk_topn_noalloc", "mdk_dock_range_timed", "mdk_dock_grid",
+    "mdk_topn_start", "mdk_topn_finish",
 ]
 # every symbol declared in include/mdk_host.h
 HOST_SYMBOLS = [
@@ -188,6 +189,17 @@ class Context:
         out = np.zeros(n, SORTSCORE_DTYPE)
         _check(lib().mdk_topn(self.h, C.c_int(nrot), C.c_int(n), C.c_void_p(out.ctypes.data)), "mdk_topn")
         return out
+
+    def topn_start(self, nrot, n):
+        """Queue the merge of rotations [0, nrot) behind the batches queued so far."""
+        _check(lib().mdk_topn_start(self.h, C.c_int(nrot), C.c_int(n)), "mdk_topn_start")
+        self._topn_n = min(n, nrot * self.t)
+
+    def topn_finish(self):
+        """Wait for the started merge; its best min(n, nrot * t) results."""
+        out = np.zeros(max(1, getattr(self, "_topn_n", 0)), SORTSCORE_DTYPE)
+        _check(lib().mdk_topn_finish(self.h, C.c_void_p(out.ctypes.data)), "mdk_topn_finish")
+        return out[:getattr(self, "_topn_n", 0)]
 
     def dock_range_timed(self, first, count):
         """dock_range + sync with HIP events around every stage of every batch: returns

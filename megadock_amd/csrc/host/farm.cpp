@@ -11,16 +11,16 @@
 // Instead of one process per job over MPI ranks, one process keeps the GPU busy:
 //   prep threads (-pj, default min(8, cores/2)) do each job's CPU work: command line,
 //     PDB reading, grid sizing, the receptor's shape/electrostatics grids (cached per
-//     receptor file and grid, since a table repeats receptors), rotation coefficients;
+//     receptor file and grid, since a table repeats receptors);
 //   the calling thread docks the prepared jobs on GPU 0 through libmdk contexts kept per
 //     FFT size (re-created only when a job needs more ligand atoms, another t or another
-//     batch; at most six alive, the least recently used freed first), taking first a
-//     prepared job whose size has a live context and differs from the job in flight.  One
-//     job is in flight while the next is set up: the next job's receptor, ligand and
-//     batches are queued on its own context before the GPU thread waits for the previous
-//     job's merge and writes its .out file, so the GPU runs one job's batches straight
-//     after the other's (a job of the in-flight job's size waits for it, as they share a
-//     context).
+//     batch; alive within a device-memory budget, -gm, the least recently used freed
+//     first), taking first a prepared job whose size has a live context and differs from
+//     the jobs in flight.  Two jobs are in flight while the next is set up: its receptor,
+//     ligand and batches are queued on its own context before the GPU thread waits for the
+//     oldest job's merge and writes its .out file, so the GPU runs the jobs' batches back
+//     to back (a job of an in-flight job's size waits for that job, as they share a
+//     context; the other one keeps the GPU busy meanwhile).
 // Host memory stays bounded on long tables: at most two prepared jobs per preparation
 // thread wait for the GPU, and the receptor grid cache keeps the grids of the last 16
 // receptors (and of any receptor a waiting job still uses).
@@ -35,6 +35,7 @@
 #include <cstring>
 #include <deque>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -262,7 +263,7 @@ struct Job {
     std::unique_ptr<Receptor> rec;
     std::unique_ptr<Ligand> lig;
     int ng = 0;
-    std::vector<float> gc, rotcoef;
+    std::vector<float> gc;
     std::shared_ptr<GridCache::Entry> grids;  // grids->grids: the receptor grids
     mdk_ctx *ctx = nullptr;  // the context the job was issued on
     std::string err;
@@ -310,11 +311,6 @@ int prepare(Job &j, GridCache &cache) {
     j.gc = grid_coord(j.ng, p.grid_width);
     j.grids = cache.get(GridCache::Key{p.rec_file, j.ng, p.grid_width, p.ace_ratio, p.rec_core, p.elec_ratio},
                         *j.rec, j.ng, p);
-    j.rotcoef.resize(9 * (size_t)p.num_rot_angles);
-    if (mdk_rotation_coef(p.zangle.data(), p.num_rot_angles, j.rotcoef.data())) {
-        j.err = std::string("mdk_rotation_coef: ") + mdk_last_error();
-        return 1;
-    }
     j.t_prep = now_s() - t0;
     return 0;
 }
@@ -330,8 +326,8 @@ struct Slot {
 
 // One context per FFT size, kept alive while their device buffers fit a budget (-gm, 160
 // GiB of the GPU's 288 GB by default; context_bytes estimates each): a table
-// with more sizes than fit frees the least recently used contexts (never the busy one,
-// whose job is in flight) before it creates another.  Freeing a context waits for the
+// with more sizes than fit frees the least recently used contexts (never one whose job is
+// in flight) before it creates another.  Freeing a context waits for the
 // whole device, so a context is made with room for ligands of up to kMinAtoms atoms and
 // is not remade for a ligand that fits it.
 class Contexts {
@@ -343,9 +339,9 @@ public:
             if (kv.second.ctx) mdk_destroy(kv.second.ctx);
     }
     // the context of the job's FFT size, able to hold its ligand, t and batch (its slot: the
-    // context and the rotation set loaded into it); `busy` (the in-flight job's context,
+    // context and the rotation set loaded into it); `busy` (the in-flight jobs' contexts,
     // another size) is never freed
-    Slot *get(const Job &j, mdk_ctx *busy, std::string &err) {
+    Slot *get(const Job &j, const std::vector<mdk_ctx *> &busy, std::string &err) {
         const int nf = 2 * j.ng, t = j.prm.num_sort;
         int batch = j.prm.batch > 0 ? j.prm.batch : auto_batch(nf, j.ng, t);
         batch = std::min(batch, std::max(1, j.prm.num_rot_angles));
@@ -385,7 +381,7 @@ public:
 private:
     // destroy the least recently used contexts other than size `keep_nf` and `busy` until
     // the others' bytes are at most `room`; returns how many were destroyed
-    int free_to(double room, int keep_nf, mdk_ctx *busy) {
+    int free_to(double room, int keep_nf, const std::vector<mdk_ctx *> &busy) {
         int freed = 0;
         for (;;) {
             double live = 0;
@@ -393,7 +389,8 @@ private:
             for (auto &kv : slots_)
                 if (kv.first != keep_nf && kv.second.ctx) {
                     live += kv.second.bytes;
-                    if (kv.second.ctx != busy && (!lru || kv.second.last_use < lru->last_use)) lru = &kv.second;
+                    const bool in_use = std::find(busy.begin(), busy.end(), kv.second.ctx) != busy.end();
+                    if (!in_use && (!lru || kv.second.last_use < lru->last_use)) lru = &kv.second;
                 }
             if (live <= room || !lru) return freed;
             mdk_destroy(lru->ctx);
@@ -412,7 +409,7 @@ private:
 // -O also scores and writes the detailed output, which needs the context.  The .out file
 // is written by write_out(), from host data only, so the GPU thread can write one job's
 // file while the next job's batches run.
-int issue(Job &j, Contexts &cs, mdk_ctx *busy) {
+int issue(Job &j, Contexts &cs, const std::vector<mdk_ctx *> &busy) {
     const double t0 = now_s();
     Parameter &p = j.prm;
     Slot *slot = cs.get(j, busy, j.err);
@@ -423,8 +420,10 @@ int issue(Job &j, Contexts &cs, mdk_ctx *busy) {
         return 1;
     };
     int &loaded = slot->rot_set;
-    if (loaded != p.rotation_set) {
-        if (mdk_set_rotations(ctx, j.rotcoef.data(), p.num_rot_angles)) return abi("mdk_set_rotations");
+    if (loaded != p.rotation_set) {  // the coefficients are made here, not kept by every waiting job
+        std::vector<float> rotcoef(9 * (size_t)p.num_rot_angles);
+        if (mdk_rotation_coef(p.zangle.data(), p.num_rot_angles, rotcoef.data())) return abi("mdk_rotation_coef");
+        if (mdk_set_rotations(ctx, rotcoef.data(), p.num_rot_angles)) return abi("mdk_set_rotations");
         loaded = p.rotation_set;
     }
     const Grids &g = *j.grids->grids;
@@ -434,18 +433,20 @@ int issue(Job &j, Contexts &cs, mdk_ctx *busy) {
                              j.gc.data(), p.grid_width, p.lig_core))
         return abi("mdk_set_ligand_atoms");
     if (mdk_dock_range(ctx, 0, p.num_rot_angles)) return abi("mdk_dock_range");
+    // the merge is queued right behind the job's last batch, before later jobs' work
+    if (mdk_topn_start(ctx, p.num_rot_angles, p.num_output)) return abi("mdk_topn_start");
     j.t_dock = now_s() - t0;
     return 0;
 }
 
-// waits for the job's merged top-N on the context issue() used (still alive: Contexts
-// never frees the busy context)
+// waits for the job's merged top-N, started by issue() on the context it used (still
+// alive: Contexts never frees a busy context)
 int complete(Job &j) {
     const double t0 = now_s();
     Parameter &p = j.prm;
     std::vector<mdk_sortscore> best(std::max(1, p.num_output));
-    if (mdk_topn(j.ctx, p.num_rot_angles, p.num_output, best.data())) {
-        j.err = std::string("mdk_topn: ") + mdk_last_error();
+    if (mdk_topn_finish(j.ctx, best.data())) {
+        j.err = std::string("mdk_topn_finish: ") + mdk_last_error();
         return 1;
     }
     j.top.resize(best.size());
@@ -485,27 +486,19 @@ public:
         space_.notify_one();
         return true;
     }
-    // without waiting: false when the queue is empty, else as pop_prefer
-    template <class P> bool try_pop_prefer(T &v, P pred) {
+    // without waiting: false when the queue is empty, else as pop_best
+    template <class R> bool try_pop_best(T &v, R rank) {
         std::lock_guard<std::mutex> g(mu_);
         if (q_.empty()) return false;
-        auto it = std::find_if(q_.begin(), q_.end(), pred);
-        if (it == q_.end()) it = q_.begin();
-        v = std::move(*it);
-        q_.erase(it);
-        space_.notify_one();
+        take_best(v, rank);
         return true;
     }
-    // as pop, but takes the first item for which pred holds when there is one
-    template <class P> bool pop_prefer(T &v, P pred) {
+    // as pop, but takes the item of lowest rank(item), the first of equal ones
+    template <class R> bool pop_best(T &v, R rank) {
         std::unique_lock<std::mutex> g(mu_);
         cv_.wait(g, [&] { return !q_.empty() || closed_; });
         if (q_.empty()) return false;
-        auto it = std::find_if(q_.begin(), q_.end(), pred);
-        if (it == q_.end()) it = q_.begin();
-        v = std::move(*it);
-        q_.erase(it);
-        space_.notify_one();
+        take_best(v, rank);
         return true;
     }
     void close() {
@@ -523,6 +516,20 @@ public:
     }
 
 private:
+    template <class R> void take_best(T &v, R rank) {  // q_ not empty, mu_ held
+        auto best = q_.begin();
+        int rb = rank(*best);
+        for (auto it = std::next(q_.begin()); it != q_.end() && rb > 0; ++it) {
+            const int r = rank(*it);
+            if (r < rb) {
+                best = it;
+                rb = r;
+            }
+        }
+        v = std::move(*best);
+        q_.erase(best);
+        space_.notify_one();
+    }
     size_t cap_, high_ = 0;
     std::mutex mu_;
     std::condition_variable cv_, space_;
@@ -575,7 +582,8 @@ int run_table(int argc, char **argv, std::string &err) {
     int nprep = o.prep_threads;
     if (nprep < 1) nprep = std::max(1, std::min(8, (int)std::thread::hardware_concurrency() / 2));
 
-    Queue<std::unique_ptr<Job>> todo, ready(2 * (size_t)nprep);  // prepared jobs wait in a bounded queue
+    const int wait_limit = 2 * nprep;  // prepared jobs wait in a bounded queue
+    Queue<std::unique_ptr<Job>> todo, ready((size_t)wait_limit);
     std::mutex rec_mu;
     int outstanding = njobs;  // jobs neither finished nor given up (guarded by rec_mu)
     for (int i = 0; i < njobs; i++) {
@@ -639,67 +647,83 @@ int run_table(int argc, char **argv, std::string &err) {
     int drain_empty = 0, drain_same = 0, ctx_made = 0, ctx_freed = 0;  // the GPU's waits between jobs
     {
         Contexts cs(o.context_gib * 1024 * 1024 * 1024);
-        // inflight: issued, its merge not yet waited for; written: merged, its .out file
-        // still to write
-        std::unique_ptr<Job> j, inflight, written;
+        // in flight: issued jobs whose merge is not yet waited for, oldest first, each on
+        // the context of its own FFT size; written: merged, their .out files still to write
+        constexpr size_t kDepth = 2;  // jobs queued on the GPU while the farm waits for a merge
+        std::deque<std::unique_ptr<Job>> inflight;
+        std::vector<std::unique_ptr<Job>> written;
+        std::unique_ptr<Job> j;
         auto report = [&](std::unique_ptr<Job> x, bool ok) {
             std::printf("# job %05d (%s): %s\n", x->id, x->line.c_str(), ok ? "done" : x->err.c_str());
             std::fflush(stdout);
             finish(std::move(x), ok);
         };
         auto write_pending = [&] {
-            if (!written) return;
+            for (auto &w : written) {
+                bool ok = false;
+                try {
+                    ok = write_out(*w) == 0;
+                } catch (const std::exception &e) {
+                    w->err = std::string("output: ") + e.what();
+                }
+                report(std::move(w), ok);
+            }
+            written.clear();
+        };
+        auto complete_oldest = [&] {
+            std::unique_ptr<Job> x = std::move(inflight.front());
+            inflight.pop_front();
             bool ok = false;
             try {
-                ok = write_out(*written) == 0;
+                ok = complete(*x) == 0;
             } catch (const std::exception &e) {
-                written->err = std::string("output: ") + e.what();
+                x->err = std::string("docking: ") + e.what();
             }
-            report(std::move(written), ok);
+            write_pending();  // the jobs merged before, if any
+            if (ok) written.push_back(std::move(x));
+            else report(std::move(x), false);
         };
-        auto complete_inflight = [&] {
-            if (!inflight) return;
-            bool ok = false;
-            try {
-                ok = complete(*inflight) == 0;
-            } catch (const std::exception &e) {
-                inflight->err = std::string("docking: ") + e.what();
-            }
-            write_pending();  // the job before, if any
-            if (ok) written = std::move(inflight);
-            else report(std::move(inflight), false);
+        auto in_flight = [&](int ng) {
+            for (auto &x : inflight)
+                if (x->ng == ng) return true;
+            return false;
         };
-        // a prepared job whose FFT size has a live context goes first (fewer contexts made
-        // and freed when the table mixes more sizes than stay alive), best of another size
-        // than the job in flight, whose batches it can then queue behind
-        auto live_other = [&](const std::unique_ptr<Job> &x) {
-            return cs.has(2 * x->ng) && (!inflight || x->ng != inflight->ng);
+        // the prepared job to take: first one of a size no job in flight has (its batches
+        // then queue behind theirs), best with a live context (fewer contexts made and freed
+        // when the table mixes more sizes than stay alive); one of an in-flight job's size
+        // only when no other is prepared
+        auto rank = [&](const std::unique_ptr<Job> &x) {
+            return in_flight(x->ng) ? 2 : cs.has(2 * x->ng) ? 0 : 1;
         };
-        auto live = [&](const std::unique_ptr<Job> &x) { return cs.has(2 * x->ng); };
         for (;;) {
-            if (!ready.try_pop_prefer(j, live_other)) {
-                if (inflight) drain_empty++;
-                complete_inflight();  // nothing prepared: finish what is queued before waiting
+            if (!ready.try_pop_best(j, rank)) {
+                if (!inflight.empty()) drain_empty++;
+                while (!inflight.empty()) complete_oldest();  // nothing prepared: finish what is queued before waiting
                 write_pending();
-                if (!ready.pop_prefer(j, live)) break;
+                if (!ready.pop_best(j, rank)) break;
             }
-            // a job of the in-flight job's size shares its context: that job completes first
-            if (inflight && inflight->ng == j->ng) {
-                drain_same++;
-                complete_inflight();
+            // a job of an in-flight job's size shares its context: that job (and the ones
+            // before it) complete first; the GPU drains only if it was the newest
+            if (in_flight(j->ng)) {
+                while (in_flight(j->ng)) complete_oldest();
+                if (inflight.empty()) drain_same++;
             }
+            std::vector<mdk_ctx *> busy;
+            for (auto &x : inflight) busy.push_back(x->ctx);
             bool ok = false;
             try {
-                ok = issue(*j, cs, inflight ? inflight->ctx : nullptr) == 0;  // queued behind the job in flight
+                ok = issue(*j, cs, busy) == 0;  // queued behind the jobs in flight
             } catch (const std::exception &e) {
                 j->err = std::string("docking: ") + e.what();
             }
-            complete_inflight();  // and its .out file is written while this job's batches run
+            // wait for the oldest merge while the newer jobs' batches run; its .out file is
+            // written then too
+            while (inflight.size() >= kDepth) complete_oldest();
             write_pending();
-            if (ok) inflight = std::move(j);
+            if (ok) inflight.push_back(std::move(j));
             else report(std::move(j), false);
         }
-        complete_inflight();
+        while (!inflight.empty()) complete_oldest();
         write_pending();
         ctx_made = cs.created;
         ctx_freed = cs.destroyed;
@@ -722,7 +746,7 @@ int run_table(int argc, char **argv, std::string &err) {
         else rotations += r.rotations;
     }
     const double el = now_s() - t_start;
-    log << "\nPrepared jobs waiting for the GPU: at most " << ready.high_water() << " (limit " << 2 * nprep << ")";
+    log << "\nPrepared jobs waiting for the GPU: at most " << ready.high_water() << " (limit " << wait_limit << ")";
     log << "\nGPU drained between jobs: " << drain_empty << " with no job prepared, " << drain_same
         << " for a job of the running job's FFT size; contexts made " << ctx_made << ", freed " << ctx_freed;
     log << "\nJobs done = " << njobs - failed << " / " << njobs << "\nElapsed time  = " << el << " sec.\n"

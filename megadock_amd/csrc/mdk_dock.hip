@@ -484,6 +484,16 @@ struct mdk_ctx {
     void *d_topn_scratch = nullptr;      // device merge (mdk_topn)
     mdk_sortscore *d_topn_out = nullptr;
     int topn_cap = 0;
+    mdk_sortscore *h_topn = nullptr;  // pinned host copy of a started merge (mdk_topn_start)
+    int h_topn_cap = 0, topn_pending = 0;
+    hipEvent_t topn_done = nullptr;
+    // pinned staging of the receptor grids and of the ligand's atom arrays: the set-up calls
+    // copy from them on the context's stream, behind the work queued before; an event after
+    // each copy guards the buffer's reuse
+    float2 *h_rstage = nullptr;
+    float *h_lstage = nullptr;
+    size_t h_rstage_n = 0, h_lstage_n = 0;  // bytes
+    hipEvent_t ev_rstage = nullptr, ev_lstage = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> tev;  // mdk_dock_range_timed: MDK_NSTAGE + 1 events per batch
 };
@@ -613,8 +623,11 @@ void drop_graphs(mdk_ctx *c) {
 }
 
 // Set-up calls replace device data that queued batches read (ligand atoms, receptor
-// spectrum and grids, rotation table, results array): they first wait for every slot's
-// stream, so that a caller may change them right after an asynchronous mdk_dock_range.
+// spectrum and grids, rotation table, results array), so that a caller may change them
+// right after an asynchronous mdk_dock_range: the rotation table and a given spectrum wait
+// for every slot's stream first; the receptor grids and the ligand atoms are uploaded on
+// the context's stream behind the queued batches instead (mdk_dock_range joins its other
+// streams into it at its end).
 int wait_slots(mdk_ctx *c) {
     for (int k = 0; k < c->nslots; k++) HIPCHK(hipStreamSynchronize(c->slot[k].stream));
     return 0;
@@ -791,6 +804,12 @@ int mdk_destroy(mdk_ctx *c) {
                     c->d_recgrid, c->d_wshape, c->d_bt, c->d_gout, c->d_gpart, c->d_rpsc, c->d_btrans, c->d_dsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->h_topn) (void)hipHostFree(c->h_topn);
+    if (c->topn_done) (void)hipEventDestroy(c->topn_done);
+    if (c->h_rstage) (void)hipHostFree(c->h_rstage);
+    if (c->h_lstage) (void)hipHostFree(c->h_lstage);
+    if (c->ev_rstage) (void)hipEventDestroy(c->ev_rstage);
+    if (c->ev_lstage) (void)hipEventDestroy(c->ev_lstage);
     drop_graphs(c);
     for (int k = 0; k < mdk_ctx::kMaxSlots; k++) {
         BatchSlot &S = c->slot[k];
@@ -808,8 +827,23 @@ int mdk_destroy(mdk_ctx *c) {
     return 0;
 }
 
-// receptor spectrum from the receptor grids already in d_recgrid (src/fft_process.cpp:212-303)
-int receptor_transform(mdk_ctx *c) {
+// a pinned staging buffer of at least `bytes` whose previous copy has completed
+int staging(void **buf, size_t &cap, size_t bytes, hipEvent_t &ev) {
+    if (ev) HIPCHK(hipEventSynchronize(ev));  // its last copy is done (long since, as a rule)
+    else HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    if (bytes > cap) {
+        if (*buf) HIPCHK(hipHostFree(*buf));
+        *buf = nullptr;
+        cap = 0;
+        HIPCHK(hipHostMalloc(buf, bytes, hipHostMallocDefault));
+        cap = bytes;
+    }
+    return 0;
+}
+
+// receptor spectrum from the receptor grids already in d_recgrid (src/fft_process.cpp:212-303);
+// `wait`: return after it is done, else with it queued on the context's stream
+int receptor_transform(mdk_ctx *c, bool wait = true) {
     const int ng = c->ng, nf = c->nf, nd = c->nd;
     const size_t ng3 = (size_t)ng * ng * ng;
     HIPCHK(hipMemcpyAsync(c->slot[0].d_lg, c->d_recgrid, sizeof(float2) * ng3, hipMemcpyDeviceToDevice, c->stream));
@@ -818,21 +852,26 @@ int receptor_transform(mdk_ctx *c) {
     k_fwd_y<<<dim3((nf + WY - 1) / WY, ng, 1), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a1, c->slot[0].d_a2, c->plan, c->d_tw, ng, nd);
     k_fwd_x_plain<<<dim3((nf + WY - 1) / WY, nf), NT, lds_bytes(c, WY), c->stream>>>(c->slot[0].d_a2, c->d_rec, c->plan, c->d_tw, ng, nd);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (wait) HIPCHK(hipStreamSynchronize(c->stream));
     c->have_rec = true;
     c->have_recgrid = true;
     return 0;
 }
 
+// Stream-ordered: the grids are copied to pinned staging at once (the caller may change them
+// on return), then uploaded and transformed on the context's stream behind the batches
+// queued before (mdk_dock_range joins its other streams into it at its end), without
+// waiting for them.
 int mdk_set_receptor(mdk_ctx *c, const float *grid_r, const float *grid_i) {
     if (!c || !grid_r || !grid_i) return fail("mdk_set_receptor: null argument");
     HIPCHK(hipSetDevice(c->device));
-    if (wait_slots(c)) return 1;
     const size_t ng3 = (size_t)c->ng * c->ng * c->ng;
-    std::vector<float2> h(ng3);
+    if (staging((void **)&c->h_rstage, c->h_rstage_n, sizeof(float2) * ng3, c->ev_rstage)) return 1;
+    float2 *h = c->h_rstage;
     for (size_t i = 0; i < ng3; i++) h[i] = make_float2(grid_r[i], grid_i[i]);
-    HIPCHK(hipMemcpyAsync(c->d_recgrid, h.data(), sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
-    return receptor_transform(c);
+    HIPCHK(hipMemcpyAsync(c->d_recgrid, h, sizeof(float2) * ng3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_rstage, c->stream));
+    return receptor_transform(c, false);
 }
 
 int mdk_set_receptor_atoms(mdk_ctx *c, int na, const float *xyz, const float *radius, const float *charge,
@@ -908,6 +947,7 @@ int mdk_get_receptor_spectrum(mdk_ctx *c, float *re, float *im) {
     if (!c || !re || !im) return fail("mdk_get_receptor_spectrum: null argument");
     if (!c->have_rec) return fail("mdk_get_receptor_spectrum: no receptor set");
     HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));  // a receptor transform may still be queued there
     const size_t nf3 = (size_t)c->nf * c->nf * c->nf;
     std::vector<float2> h(nf3);
     HIPCHK(hipMemcpy(h.data(), c->d_rec, sizeof(float2) * nf3, hipMemcpyDeviceToHost));
@@ -931,24 +971,31 @@ int mdk_set_ligand_atoms(mdk_ctx *c, int na, const float *xyz, const float *cent
     if (delta == 0.0f) return fail("mdk_set_ligand: core value 0 is not supported (the reference's last shape sweep is order-dependent then)");
     if (!(gw > 0.0f)) return fail("mdk_set_ligand: grid width must be positive");
     HIPCHK(hipSetDevice(c->device));
-    if (wait_slots(c)) return 1;
     drop_graphs(c);  // they hold the ligand's kernel arguments
-    std::vector<float> rc2(na), rs2(na);
+    // stream-ordered, as mdk_set_receptor: staged at once, uploaded on the context's stream
+    // behind the batches queued before
+    const size_t n = (size_t)na, ng = (size_t)c->ng;
+    if (staging((void **)&c->h_lstage, c->h_lstage_n, sizeof(float) * (7 * n + ng), c->ev_lstage)) return 1;
+    float *h_xyz = c->h_lstage, *h_rc2 = h_xyz + 3 * n, *h_rs2 = h_rc2 + n, *h_chg = h_rs2 + n,
+          *h_ws = h_chg + n, *h_gc = h_ws + n;
+    std::memcpy(h_xyz, xyz, sizeof(float) * 3 * n);
+    std::memcpy(h_chg, charge, sizeof(float) * n);
+    std::memcpy(h_gc, grid_coord, sizeof(float) * ng);
     for (int i = 0; i < na; i++) {  // src/fft_process.cpp:784-787 / src/ligand.cpp:254-257
-        rc2[i] = radius[i] * radius[i] * 1.5f;
-        rs2[i] = radius[i] * radius[i] * 1.0f;
+        h_rc2[i] = radius[i] * radius[i] * 1.5f;
+        h_rs2[i] = radius[i] * radius[i] * 1.0f;
+        // companion shape weight: the atom's volume in voxels
+        h_ws[i] = (float)(4.0 / 3.0 * M_PI * (double)radius[i] * radius[i] * radius[i] / ((double)gw * gw * gw));
     }
-    HIPCHK(hipMemcpy(c->d_xyz, xyz, sizeof(float) * 3 * na, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_rc2, rc2.data(), sizeof(float) * na, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_rs2, rs2.data(), sizeof(float) * na, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_charge, charge, sizeof(float) * na, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_gc, grid_coord, sizeof(float) * c->ng, hipMemcpyHostToDevice));
-    {
-        std::vector<float> ws(na);  // companion shape weight: the atom's volume in voxels
-        for (int i = 0; i < na; i++)
-            ws[i] = (float)(4.0 / 3.0 * M_PI * (double)radius[i] * radius[i] * radius[i] / ((double)gw * gw * gw));
-        HIPCHK(hipMemcpy(c->d_wshape, ws.data(), sizeof(float) * na, hipMemcpyHostToDevice));
-    }
+    const struct {
+        float *dst;
+        const float *src;
+        size_t n;
+    } up[] = {{c->d_xyz, h_xyz, 3 * n}, {c->d_rc2, h_rc2, n}, {c->d_rs2, h_rs2, n},
+              {c->d_charge, h_chg, n}, {c->d_wshape, h_ws, n}, {c->d_gc, h_gc, ng}};
+    for (const auto &u : up)
+        HIPCHK(hipMemcpyAsync(u.dst, u.src, sizeof(float) * u.n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev_lstage, c->stream));
     LigandDev &L = c->lig;
     L.na = na;
     L.ng = c->ng;
@@ -1075,9 +1122,12 @@ int mdk_get_results(mdk_ctx *c, int first, int count, mdk_sortscore *out) {
     return 0;
 }
 
-int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
-    if (!c || !out || !c->d_results) return fail("mdk_topn: nothing to merge");
-    if (nrot < 1 || nrot > c->nrot || n < 1) return fail("mdk_topn: invalid count");
+// the merge on the context's stream, after every batch queued so far (mdk_dock_range
+// joins its slots' streams into it), its n best copied to the pinned host buffer
+int topn_enqueue(mdk_ctx *c, int nrot, int n, const char *who) {
+    if (!c || !c->d_results) return fail(std::string(who) + ": nothing to merge");
+    if (nrot < 1 || nrot > c->nrot || n < 1) return fail(std::string(who) + ": invalid count");
+    if (c->topn_pending) return fail(std::string(who) + ": a started merge was not finished (mdk_topn_finish)");
     HIPCHK(hipSetDevice(c->device));
     // on the device: order (score desc, rotation asc, rank asc), mdk_merge.hip
     const size_t m = (size_t)nrot * c->t;
@@ -1092,10 +1142,38 @@ int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
         HIPCHK(hipMalloc((void **)&c->d_topn_out, sizeof(mdk_sortscore) * keep));
         c->topn_cap = keep;
     }
+    if (keep > c->h_topn_cap) {  // no merge of this context is in flight (topn_pending is 0)
+        if (c->h_topn) HIPCHK(hipHostFree(c->h_topn));
+        c->h_topn = nullptr;
+        c->h_topn_cap = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_topn, sizeof(mdk_sortscore) * keep, hipHostMallocDefault));
+        c->h_topn_cap = keep;
+    }
+    if (!c->topn_done) HIPCHK(hipEventCreateWithFlags(&c->topn_done, hipEventDisableTiming));
     HIPCHK(launch_topn(c->d_results, (unsigned)m, (unsigned)keep, c->d_topn_scratch, c->d_topn_out, c->stream));
-    HIPCHK(hipMemcpyAsync(out, c->d_topn_out, sizeof(mdk_sortscore) * keep, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_topn, c->d_topn_out, sizeof(mdk_sortscore) * keep, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->topn_done, c->stream));
+    c->topn_pending = keep;
     return 0;
+}
+
+int mdk_topn_start(mdk_ctx *c, int nrot, int n) { return topn_enqueue(c, nrot, n, "mdk_topn_start"); }
+
+int mdk_topn_finish(mdk_ctx *c, mdk_sortscore *out) {
+    if (!c || !out) return fail("mdk_topn_finish: null argument");
+    if (!c->topn_pending) return fail("mdk_topn_finish: no merge was started (mdk_topn_start)");
+    const int keep = c->topn_pending;
+    c->topn_pending = 0;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventSynchronize(c->topn_done));
+    std::memcpy(out, c->h_topn, sizeof(mdk_sortscore) * keep);
+    return 0;
+}
+
+int mdk_topn(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {
+    if (!out) return fail("mdk_topn: null output");
+    if (topn_enqueue(c, nrot, n, "mdk_topn")) return 1;
+    return mdk_topn_finish(c, out);
 }
 
 int mdk_topn_noalloc(mdk_ctx *c, int nrot, int n, mdk_sortscore *out) {

@@ -21,10 +21,10 @@ struct Defaults {
 };
 constexpr Defaults kDefaults[] = {
     {64, 7, 3, 0}, {66, 7, 3, 2}, {70, 7, 3, 0}, {72, 7, 3, 0}, {80, 6, 3, 0}, {84, 6, 3, 0},
-    {88, 6, 3, 0}, {90, 6, 2, 0}, {96, 5, 3, 0}, {98, 7, 3, 0}, {100, 6, 3, 0}, {108, 5, 2, 0},
-    {110, 7, 3, 0}, {112, 6, 3, 0}, {120, 5, 3, 0}, {126, 7, 2, 0}, {128, 5, 2, 0}, {132, 7, 3, 0},
-    {140, 6, 3, 0}, {144, 6, 3, 0}, {150, 5, 2, 0}, {154, 6, 2, 2}, {160, 7, 2, 0}, {162, 5, 2, 2},
-    {168, 6, 2, 2}, {176, 6, 2, 0}, {180, 6, 2, 2}, {192, 7, 2, 2}, {256, 6, 2, 0},
+    {88, 6, 3, 0}, {90, 6, 2, 0}, {96, 7, 3, 0}, {98, 7, 3, 0}, {100, 6, 3, 0}, {108, 5, 3, 0},
+    {110, 7, 3, 0}, {112, 6, 3, 0}, {120, 5, 3, 0}, {126, 6, 3, 0}, {128, 5, 2, 0}, {132, 7, 3, 0},
+    {140, 6, 3, 0}, {144, 6, 3, 0}, {150, 5, 2, 0}, {154, 6, 2, 0}, {160, 7, 2, 0}, {162, 5, 2, 0},
+    {168, 7, 2, 0}, {176, 5, 2, 0}, {180, 6, 2, 2}, {192, 6, 2, 2}, {256, 6, 2, 0},
 };
 template <int N> constexpr int further_inv_default() { return N <= 128 && inv_prefetch_fits<N>() ? 3 : 2; }
 }  // namespace

@@ -3,6 +3,7 @@ counters).
 
     python -m megadock_amd.profiling trace DIR           # --kernel-trace --stats, -f csv
     python -m megadock_amd.profiling pmc --batch B --json OUT DIR [DIR ...]
+    python -m megadock_amd.profiling pmc-md OUT           # the tables of a written summary
 
 DIR are rocprofv3 output folders written with `-f csv -o run`.  `pmc` reads the counter
 passes of one command (FETCH_SIZE in one pass, WRITE_SIZE in another, SQ counters in
@@ -164,10 +165,27 @@ def pmc_summary(dirs, batch, copy_bytes=1 << 30, min_dispatches=4):
             row["write_bytes"] = cs["WRITE_SIZE"][1] * 1024 * factors["WRITE_SIZE"]
             row["hbm_bytes"] = row["fetch_bytes"] + row["write_bytes"]
         kernels[k] = row
-    out = {"batch": batch, "copy_calibration": {"bytes": copy_bytes, "factors": factors}, "kernels": kernels}
-    if kernels and all("hbm_bytes" in v for v in kernels.values()):
-        out["pipeline_hbm_bytes"] = sum(v["hbm_bytes"] for v in kernels.values())
+    out = {"batch": batch, "copy_calibration": {"bytes": copy_bytes, "factors": factors},
+           "kernels": timed_pass_kernels(kernels)}
+    finish_pipeline(out)
     return out
+
+
+def timed_pass_kernels(kernels):
+    """The kernels of the timed passes: those dispatched at least half as often as the most
+    dispatched one.  The autotuner's trial batches run other forms of a stage (for example
+    the two-pass inverse next to the fused one) a few times; they do not count."""
+    if not kernels:
+        return kernels
+    top = max(v["dispatches"] for v in kernels.values())
+    return {k: v for k, v in kernels.items() if v["dispatches"] * 2 >= top}
+
+
+def finish_pipeline(out):
+    ks = out["kernels"]
+    out.pop("pipeline_hbm_bytes", None)
+    if ks and all("hbm_bytes" in v for v in ks.values()):
+        out["pipeline_hbm_bytes"] = sum(v["hbm_bytes"] for v in ks.values())
 
 
 def main(argv=None):
@@ -181,6 +199,8 @@ def main(argv=None):
     p.add_argument("--batch", type=int, required=True)
     p.add_argument("--copy-bytes", type=int, default=1 << 30)
     p.add_argument("--json", required=True)
+    m = sub.add_parser("pmc-md", help="the tables of a summary JSON written by pmc")
+    m.add_argument("json")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print("rocprofv3 --stats summary (all dispatches):\n")
@@ -198,22 +218,31 @@ def main(argv=None):
             b = busy(a.dir)
             print("\nGPU busy over the docking span (union of the kernels' intervals): %s" % json.dumps(b))
         return 0
+    if a.cmd == "pmc-md":
+        print_pmc(json.load(open(a.json)))
+        return 0
     s = pmc_summary(a.dirs, a.batch, a.copy_bytes)
     with open(a.json, "w") as fh:
         json.dump(s, fh, indent=1)
+    print_pmc(s)
+    return 0
+
+
+def print_pmc(s):
+    """The markdown tables of a counter summary (pmc_summary's dict)."""
+    batch = s["batch"]
     f = s["copy_calibration"]["factors"]
     print("Copy calibration (%d bytes copied): FETCH_SIZE x %s, WRITE_SIZE x %s\n"
           % (s["copy_calibration"]["bytes"], f["FETCH_SIZE"], f["WRITE_SIZE"]))
     cols = sorted({c for v in s["kernels"].values() for c in v if c.isupper() or c.startswith("SQ_")})
-    print("Per full-batch dispatch (%d rotations), mean over the dispatches of the largest grid:\n" % a.batch)
+    print("Per full-batch dispatch (%d rotations), mean over the dispatches of the largest grid:\n" % batch)
     print("| kernel | dispatches | HBM MB | " + " | ".join(cols) + " |\n|---|---:|---:|" + "---:|" * len(cols))
     for k, v in sorted(s["kernels"].items(), key=lambda kv: -kv[1].get("hbm_bytes", 0)):
         hb = "%.1f" % (v["hbm_bytes"] / 1e6) if "hbm_bytes" in v else "-"
         print("| %s | %d | %s | " % (k, v["dispatches"], hb) + " | ".join("%.4g" % v[c] if c in v else "-" for c in cols) + " |")
     if "pipeline_hbm_bytes" in s:
         print("\nPipeline: %.1f MB per batch, %.2f MB per rotation"
-              % (s["pipeline_hbm_bytes"] / 1e6, s["pipeline_hbm_bytes"] / a.batch / 1e6))
-    return 0
+              % (s["pipeline_hbm_bytes"] / 1e6, s["pipeline_hbm_bytes"] / batch / 1e6))
 
 
 if __name__ == "__main__":

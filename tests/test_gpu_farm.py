@@ -103,10 +103,11 @@ def _farm_peak_rss_kib(table, log):
 
 def test_farm_host_memory_stays_bounded_on_a_long_table(tmp_path, golden_dir):
     """A table on which the GPU is the slower stage (the 54000-rotation set: 0.2 s of GPU
-    work per job, and 2.6 MB of rotation angles and coefficients per prepared job): the
-    preparation threads stop at two waiting jobs each, so the queue of prepared jobs never
-    holds more than that, and the farm's peak memory does not grow with the number of jobs
-    (an unbounded queue would hold every job of the longer table, about 200 MB more)."""
+    work per job, and 0.6 MB of rotation angles per prepared job; the rotation coefficients
+    are made only when a context's table changes): the preparation threads stop at two
+    waiting jobs each, so the queue of prepared jobs never holds more than that, and the
+    farm's peak memory does not grow with the number of jobs (an unbounded queue would hold
+    every job of the longer table)."""
     r = os.path.join(golden_dir, "1gcq_r.pdb")
     l = os.path.join(golden_dir, "1gcq_l.pdb")
     peak = {}

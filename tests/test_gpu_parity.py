@@ -340,6 +340,34 @@ def test_topn_device_merge(gpu, gcq, nrot, t, n):
     ctx.close()
 
 
+def test_topn_started_then_finished_equals_topn(gpu, gcq):
+    """mdk_topn_start queues the merge behind the queued batches and returns; work of
+    another context queued meanwhile does not change it; mdk_topn_finish returns what
+    mdk_topn returns.  A second start before the finish, and a finish without a start,
+    are errors."""
+    rec, lig, ng = gcq
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rc = M.rotation_coefs(O.load_rotations(3600)[:300])
+    a, b = _lig_ctx(lig, ng, t=2, max_batch=64), _lig_ctx(lig, ng, t=1, max_batch=32)
+    for c in (a, b):
+        c.set_receptor(gr, gi)
+        c.set_rotations(rc)
+    _say("started merge over 300 rotations, another context queued behind it")
+    a.dock_range(0, 300)
+    a.topn_start(300, 200)
+    with pytest.raises(M.MdkError):
+        a.topn_start(300, 200)
+    b.dock_range(0, 300)
+    started = a.topn_finish()
+    with pytest.raises(M.MdkError):
+        a.topn_finish()
+    assert len(started) == 200
+    assert np.array_equal(started.view(np.uint8), a.topn(300, 200).view(np.uint8))
+    b.sync()
+    a.close()
+    b.close()
+
+
 def test_topn_device_merge_all_tied(gpu, gcq):
     """Every score 0 (zero receptor spectrum): the merge keeps result order."""
     rec, lig, ng = gcq
