@@ -1263,10 +1263,20 @@ template <int N, int LEVEL> hipError_t run_pipe(const PipeArgs &a) {
     // of 4 rotations (1); one rotation per workgroup, tile index fastest (2); as 6 without
     // the padding columns (7)
     constexpr int LC = 2 * LX, NCH = (N / 2 + 1) / LC, NFL = ((N / 2 + 1) % LC * N + LC - 1) / LC;
-    if (perm) {  // the parity-split d_a2: variant 6's tiling
-        if constexpr (fwd_half_fits<N>())
-            kp_x_corr<N, 2 * LX, 1, true, true, true, false, false, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
-                a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+    if (perm) {  // the parity-split d_a2: variant 6's tiling, or 7's (whole chunks and a flat tail)
+        if constexpr (fwd_half_fits<N>()) {
+            if (a.variant == 7 && NFL > 0) {
+                if constexpr (NCH > 0)
+                    kp_x_corr<N, LC, 1, true, true, true, false, false, true><<<dim3(B, NCH, N), 2 * LC * T, 0, s>>>(
+                        a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+                if constexpr (NFL > 0)
+                    kp_x_corr<N, LC, 1, true, true, true, false, true, true><<<dim3(B, NFL, 1), 2 * LC * T, 0, s>>>(
+                        a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+            } else {
+                kp_x_corr<N, 2 * LX, 1, true, true, true, false, false, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
+                    a.a2, a.rec, a.cc, a.tw4, a.H, a.Hp, B);
+            }
+        }
     } else if constexpr (!FULL) {
         if (a.variant == 5) {
             kp_x_corr<N, 2 * LX, 1, false, true, true><<<dim3(B, a.Hp / (2 * LX), N), 4 * LX * T, 0, s>>>(
