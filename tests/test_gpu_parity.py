@@ -368,6 +368,48 @@ def test_topn_started_then_finished_equals_topn(gpu, gcq):
     b.close()
 
 
+def test_setup_calls_queue_behind_docking(gpu, gcq, synth):
+    """mdk_set_receptor and mdk_set_ligand_atoms return without waiting for the batches
+    queued before them: those batches see the previous receptor and ligand, the batches
+    queued after see the new ones, bit for bit as contexts set up once with each pair."""
+    rec, lig, ng = gcq
+    _, lig2 = synth
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    gr2 = np.roll(gr.reshape(ng, ng, ng), 5, axis=0).ravel().copy()
+    gi2 = np.roll(gi.reshape(ng, ng, ng), 5, axis=0).ravel().copy()
+    rc = M.rotation_coefs(O.load_rotations(3600)[:300])
+    gc = O.grid_coord(ng, 1.2)
+
+    def ctx_with(r, l):
+        ctx = M.Context(2 * ng, ng, lig.na, t=2, max_batch=64)
+        ctx.set_receptor(*r)
+        ctx.set_ligand(l.xyz, l.center, l.radius, l.charge, gc, 1.2)
+        ctx.set_rotations(rc)
+        return ctx
+
+    refs = []
+    for r, l in (((gr, gi), lig), ((gr2, gi2), lig), ((gr2, gi2), lig2)):
+        ctx = ctx_with(r, l)
+        ctx.dock_range(0, 300)
+        ctx.sync()
+        refs.append(ctx.results(0, 300))
+        ctx.close()
+    _say("set-up calls between queued ranges")
+    ctx = ctx_with((gr, gi), lig)
+    ctx.dock_range(0, 100)
+    ctx.set_receptor(gr2, gi2)  # queued behind rotations 0..99
+    ctx.dock_range(100, 100)
+    ctx.set_ligand(lig2.xyz, lig2.center, lig2.radius, lig2.charge, gc, 1.2)  # behind 100..199
+    ctx.dock_range(200, 100)
+    ctx.sync()
+    got = ctx.results(0, 300)
+    ctx.close()
+    for k in range(3):
+        sl = slice(200 * k, 200 * (k + 1))  # t = 2 results per rotation
+        assert np.array_equal(got[sl].view(np.uint8), refs[k][sl].view(np.uint8)), k
+    assert not np.array_equal(refs[0][:200].view(np.uint8), refs[1][:200].view(np.uint8))
+
+
 def test_topn_device_merge_all_tied(gpu, gcq):
     """Every score 0 (zero receptor spectrum): the merge keeps result order."""
     rec, lig, ng = gcq
