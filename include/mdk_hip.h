@@ -203,9 +203,9 @@ int mdk_autotune(mdk_ctx *ctx, int first, int count, int *choice);
  * t[3b..3b+2] in voxels, the units of a result's index[1..3].  out[4b..4b+3] =
  * (s, ds/dtx, ds/dty, ds/dtz) where s is the band-limited interpolation of the hard-voxel
  * score map (value / nf^3, equal to the reported score at grid points).  Needs an FFT
- * size with specialised kernels: 64-192 from the reference's size table (see
- * megadock_amd/csrc/mdk_pipe.hip) and 256.  Any B >= 0; runs in batches of the
- * context's max_batch. */
+ * size with specialised kernels: every one from 64 to 320 of the reference's size table
+ * (see megadock_amd/csrc/mdk_pipe.hip).  Any B >= 0; runs in batches of the context's
+ * max_batch. */
 int mdk_score_spectral(mdk_ctx *ctx, const float *rotcoef, const float *t, int B, float *out);
 
 /* Companion score of B poses (same pose arguments) with the ligand atoms spread

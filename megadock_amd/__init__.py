@@ -37,7 +37,8 @@ STAGES = ["voxelize", "fwd_z", "fwd_y", "x_corr", "inv_y", "inv_z_top", "reduce"
 # other sizes run the generic mixed-radix kernels
 TUNED_NF = [80, 96, 144, 256]
 FURTHER_NF = [64, 66, 70, 72, 84, 88, 90, 98, 100, 108, 110, 112, 120, 126, 128, 132, 140, 150, 154, 160, 162, 168,
-              176, 180, 192]
+              176, 180, 192, 196, 198, 200, 210, 216, 220, 224, 240, 242, 250, 252, 264, 270, 280, 288, 294, 300, 308,
+              320]
 
 _lib = None
 

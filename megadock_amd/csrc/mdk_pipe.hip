@@ -5,7 +5,8 @@
 // further sizes (mdk_pipe_size.hip, the Makefile's PIPE_SIZES)
 #define MDK_PIPE_SIZES(X) \
     X(64) X(66) X(70) X(72) X(84) X(88) X(90) X(98) X(100) X(108) X(110) X(112) X(120) X(126) X(128) X(132) \
-    X(140) X(150) X(154) X(160) X(162) X(168) X(176) X(180) X(192)
+    X(140) X(150) X(154) X(160) X(162) X(168) X(176) X(180) X(192) X(196) X(198) X(200) X(210) X(216) X(220) \
+    X(224) X(240) X(242) X(250) X(252) X(264) X(270) X(280) X(288) X(294) X(300) X(308) X(320)
 #define MDK_DECLARE(n) hipError_t pipe_run_n##n(const PipeArgs &a);
 MDK_PIPE_SIZES(MDK_DECLARE)
 #undef MDK_DECLARE

@@ -652,20 +652,24 @@ __device__ __forceinline__ void block_top(TopR<TT> &tl, int t, Cand *wl, Cand *d
 // groups; the lists are merged across each wavefront with lane shuffles and across
 // wavefronts through LDS once per plane, leaving the plane's top-t as t candidates.  The
 // test hook writes the real score rows to smap instead.
+// The workgroup has L * N1z threads rounded up to whole wavefronts (the extra lanes only
+// take part in the wavefront merges); a last row group that runs past the plane's N rows
+// transforms zeros there and pushes nothing from them.
+template <int N, int L> constexpr int inv_z_threads() { return (L * Geo<N / 2>::N1 + 63) / 64 * 64; }
 template <int N, int L, int TT>
-__global__ void __launch_bounds__(L *Geo<N / 2>::N1)
+__global__ void __launch_bounds__((inv_z_threads<N, L>()))
     kp_inv_z_top(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                  const float2 *__restrict__ twg, int H, int Hp, int t, float nf3f) {
     constexpr int M = N / 2, HC = M + 1;
-    constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2, NT = L * N1z;
+    constexpr int N1z = Geo<M>::N1, N2z = Geo<M>::N2, NT = inv_z_threads<N, L>(), NA = L * N1z;
     constexpr int LSR = HC % 2 ? HC : HC + 1;  // row stride in LDS (odd)
-    constexpr int NROW = N * N, NW = (NT + 63) / 64, RG = N / L;
-    static_assert(NT % 64 == 0, "the wavefront top-t needs whole wavefronts");
-    static_assert(N % L == 0 && N2z <= N1z, "whole row groups per plane, one task per thread and step");
+    constexpr int NROW = N * N, NW = NT / 64, RG = (N + L - 1) / L;
+    static_assert(N2z <= N1z, "one task per thread and step");
     __shared__ float2 S[L * LSR];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
     const int tid = threadIdx.x, line = tid % L, u = tid / L;
+    const bool act = tid < NA;  // lanes beyond the row group's tasks only merge
     for (int q = tid; q < N; q += NT) tw[q] = twg[q];
     const int b = blockIdx.y;
     const float2 *src = cc + ((size_t)b * NROW) * Hp;
@@ -680,15 +684,16 @@ __global__ void __launch_bounds__(L *Geo<N / 2>::N1)
     tl.init();
     float run_tau = -INFINITY, cut = -INFINITY;
     for (int g = 0; g < RG; g++) {
-        const int row0 = (blockIdx.x * RG + g) * L;
+        const int y0 = g * L, row0 = blockIdx.x * N + y0;
+        const bool row_ok = act && y0 + line < N;
         __syncthreads();  // S is free: the previous row group has read it
         for (int q = tid; q < L * HC; q += NT) {
             const int l = q / HC, kz = q - l * HC;
-            S[l * LSR + kz] = src[(size_t)(row0 + l) * Hp + kz];
+            S[l * LSR + kz] = y0 + l < N ? src[(size_t)(row0 + l) * Hp + kz] : make_float2(0.f, 0.f);
         }
         __syncthreads();
         float2 v[N1z];
-        if (u < N2z)
+        if (act && u < N2z)
             sfor<0, N1z>([&](auto jc) {
                 constexpr int j1 = decltype(jc)::value;
                 const int k = u + N2z * j1;
@@ -700,48 +705,51 @@ __global__ void __launch_bounds__(L *Geo<N / 2>::N1)
                 v[j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
             });
         __syncthreads();  // every read of S is done before step 1 writes into it
-        if (u < N2z) rf::step1_g<M, true>(v, S, tw, 2, 1, LSR, line, u);
+        if (act && u < N2z) rf::step1_g<M, true>(v, S, tw, 2, 1, LSR, line, u);
         __syncthreads();
         float2 y[N2z];
-        rf::step2_g<M, true>(y, S, 1, LSR, line, u);
+        if (act) rf::step2_g<M, true>(y, S, 1, LSR, line, u);
         const int r = row0 + line;
         // z = 2m and 2m+1 of row r are y[k2].x and .y with m = u + N1z k2
         if (smap) {  // uniform
-            sfor<0, N2z>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                const size_t o = ((size_t)b * NROW + r) * N + 2 * (u + N1z * k2);
-                smap[o] = y[k2].x;
-                smap[o + 1] = y[k2].y;
-            });
+            if (row_ok)
+                sfor<0, N2z>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const size_t o = ((size_t)b * NROW + r) * N + 2 * (u + N1z * k2);
+                    smap[o] = y[k2].x;
+                    smap[o + 1] = y[k2].y;
+                });
             continue;
         }
         if (g < 4 || (g & 3) == 0) {  // uniform
             float mx = -INFINITY;
-            sfor<0, N2z>([&](auto kc) {
-                constexpr int k2 = decltype(kc)::value;
-                mx = fmaxf(mx, fmaxf(y[k2].x, y[k2].y));
-            });
+            if (row_ok)
+                sfor<0, N2z>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    mx = fmaxf(mx, fmaxf(y[k2].x, y[k2].y));
+                });
             const float tau = wave_kth_max(mx, t);
             if (tau > run_tau) {  // uniform
                 run_tau = tau;
                 cut = QUOT ? quotient_cut(tau, nf3f) : tau;
             }
         }
-        sfor<0, N2z>([&](auto kc) {
-            constexpr int k2 = decltype(kc)::value;
-            const int z0 = 2 * (u + N1z * k2);
-            const float v0 = y[k2].x, v1 = y[k2].y;
-            if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, r * N + z0);
-            if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, r * N + z0 + 1);
-        });
+        if (row_ok)
+            sfor<0, N2z>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                const int z0 = 2 * (u + N1z * k2);
+                const float v0 = y[k2].x, v1 = y[k2].y;
+                if (v0 >= cut) tl.push(QUOT ? v0 / nf3f : v0, r * N + z0);
+                if (v1 >= cut) tl.push(QUOT ? v1 / nf3f : v1, r * N + z0 + 1);
+            });
     }
     if (smap) return;  // uniform
     block_top<TT, NW>(tl, t, wl, cand + ((size_t)b * gridDim.x + blockIdx.x) * t);
 }
 
 template <int N, int L, int TT> void launch_inv_z(const PipeArgs &a, hipStream_t s) {
-    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), L * Geo<N / 2>::N1, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp, a.t,
-                                                                       a.nf3f);
+    kp_inv_z_top<N, L, TT><<<dim3(N, a.B), inv_z_threads<N, L>(), 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.H, a.Hp,
+                                                                          a.t, a.nf3f);
 }
 
 // the fused inverse keeps a whole (nf x (nf/2+1)) half-spectrum plane in LDS and gives
@@ -751,8 +759,7 @@ template <int N> constexpr bool fused_inverse() {
     return (size_t)N * plane_stride(N) * 8 <= 150 * 1024 && (N / 2 + 1) * Geo<N>::N1 <= 2 * 1024 &&
            (N / 2 + 1) * Geo<N>::N2 <= 2 * 1024;
 }
-// the unfused inverse walks a plane in groups of 16 rows
-template <int N> constexpr bool pipe_size_ok() { return N % 2 == 0 && (fused_inverse<N>() || N % 16 == 0); }
+template <int N> constexpr bool pipe_size_ok() { return N % 2 == 0; }
 
 // ------------------------------------------------------------------ fused inverse y + z + top-t
 // Persistent workgroups, each walking the (b, x) planes p = b*nf + x of cc (nf rows ky,
@@ -992,9 +999,10 @@ __device__ __forceinline__ float2 fold_half(float2 a, float2 c, int h, float2 w)
 }
 
 template <int N> constexpr bool half_inverse_fits() {
-    constexpr int M = N / 2;
-    return N % 4 == 0 && (size_t)M * plane_stride(N) * 8 + (size_t)N * 8 + 16 * 16 * 8 <= 150 * 1024 &&
-           Geo<M>::N1 <= 32;
+    constexpr int M = N / 2, N1 = Geo<M>::N1, N2 = Geo<M>::N2;
+    // whole columns and rows a round of 1024 threads
+    return N % 4 == 0 && (size_t)M * plane_stride(N) * 8 + (size_t)N * 8 + 16 * 16 * 8 <= 150 * 1024 && N1 <= 32 &&
+           1024 % N1 == 0 && 1024 % N2 == 0 && M % (1024 / N1) == 0 && M * N2 <= 1024;
 }
 
 template <int N, int TT>

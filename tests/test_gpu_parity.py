@@ -199,7 +199,7 @@ def test_dock_vs_oracle(gpu, gcq, synth, kernels, t, which, ng):
 
 
 @pytest.mark.parametrize("t", [1, 2])
-@pytest.mark.parametrize("nf", M.FURTHER_NF)
+@pytest.mark.parametrize("nf", [n for n in M.FURTHER_NF if n <= 192])
 def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf, t):
     """The sizes specialised with the default kernel variants only (a table farm meets
     them: nf = 2 ng from the reference's FFT size table, including ng odd, where the
@@ -232,6 +232,49 @@ def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf, t):
         assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
         if not np.array_equal(g["index"], r["index"]):
             b = int(ids[q // t])
+            xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[b]))
+            lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+            s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+            ig = (g["index"][1] * nf + g["index"][2]) * nf + g["index"][3]
+            ir = (r["index"][1] * nf + r["index"][2]) * nf + r["index"][3]
+            assert _near_tie_ok(s64, ig, ir, 2e-6), (q, g, r)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nf", [n for n in M.FURTHER_NF if n > 192])
+def test_sizes_above_192_vs_oracle(gpu, synth, nf):
+    """The table sizes above 192 (specialised with the passes that do not need a plane in
+    LDS: forward z then y through HBM, the correlation pass, inverse y in place, then the
+    z step over row groups, the last one partial where 16 does not divide nf): per-rotation
+    top-2 against the oracle on two rotations; at 200, 252 and 320 also the score map
+    against FP64."""
+    rec, lig = synth
+    ng = nf // 2
+    gr, gi = O.receptor_voxel(ng, 1.2, rec)
+    rr, ri = O.receptor_fft(ng, nf, gr, gi)
+    ctx = _lig_ctx(lig, ng, t=2, max_batch=4)
+    ctx.set_receptor(gr, gi)
+    ang = O.load_rotations(3600)
+    remb = O.embed(ng, nf, gr, gi)
+    if nf in (200, 252, 320):
+        _say("size nf=%d: score map" % nf)
+        a = 2345
+        smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
+        xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[a]))
+        lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
+        ref = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
+        assert np.abs(smap - ref).max() <= 2e-6 * np.abs(ref).max()
+        del smap, ref
+    ids = np.array([11, 2000], dtype=np.int32)
+    _say("size nf=%d: dock %d rotations" % (nf, len(ids)))
+    got = ctx.dock(M.rotation_coefs(ang[ids]), ids)
+    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, 2, threads=16)
+    for q in range(len(ids) * 2):
+        g, r = got[q], exp[q]
+        assert g["index"][0] == r["index"][0]
+        assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
+        if not np.array_equal(g["index"], r["index"]):
+            b = int(ids[q // 2])
             xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[b]))
             lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
             s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()

@@ -60,7 +60,7 @@ def test_missing_table_is_an_error(tmp_path):
 
 def test_specialised_size_lists_match_the_build():
     """The Python lists of specialised sizes are the ones the library is built with."""
-    mk = open(os.path.join(ROOT, "megadock_amd", "csrc", "Makefile")).read()
+    mk = open(os.path.join(ROOT, "megadock_amd", "csrc", "Makefile")).read().replace("\\\n", " ")
     sizes = [int(x) for x in re.search(r"^PIPE_SIZES := (.*)$", mk, re.M).group(1).split()]
     assert sizes == M.FURTHER_NF
     src = open(os.path.join(ROOT, "megadock_amd", "csrc", "mdk_pipe.hip")).read()
