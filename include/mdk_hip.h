@@ -191,8 +191,8 @@ int mdk_score_map(mdk_ctx *ctx, const float *rotcoef, float *out);
  * prefetch or split) on rotations [first, first+count) of the table and keeps the fastest
  * for this context; choice[0..2] receives the chosen (correlation, inverse, forward)
  * variants.  The variants give bitwise-identical results, except at nf = 256, where the
- * two-pass forms (inverse variant 0, forward variant 1) round differently from the
- * half-plane fused passes.  The sizes 80, 96 and 144 have every variant, 256 correlation
+ * two-pass forms (inverse variant 0, forward variant 1), the half-plane fused passes and
+ * the 512-thread half-plane inverse (variant 1) round differently.  The sizes 80, 96 and 144 have every variant, 256 correlation
  * 0-7 (6 with the fused forward), inverse 0-3 and forward 0-1, the other specialised sizes
  * correlation 5-7, inverse 2-3 and forward 0 and 2.  No reference counterpart. */
 int mdk_autotune(mdk_ctx *ctx, int first, int count, int *choice);

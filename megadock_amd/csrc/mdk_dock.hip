@@ -1320,8 +1320,8 @@ int mdk_autotune(mdk_ctx *c, int first, int count, int *choice) {
         // the variants built for this size: correlation x inverse first, then the forward
         // pass; each timed twice after one warm-up pass.  Every variant computes the same
         // results (same operations in the same order), except at nf = 256, where the two-pass
-        // forms (inverse variant 0, forward variant 1) and the half-plane fused passes round
-        // differently
+        // forms (inverse variant 0, forward variant 1), the half-plane fused passes and the
+        // 512-thread half-plane inverse (variant 1) round differently
         auto time_it = [&](float &ms) {
             drop_graphs(c);
             return mdk_dock_range(c, first, count) || mdk_time_range(c, first, count, 2, &ms);

@@ -282,7 +282,8 @@ struct PipeArgs {
     int cu_count;       // compute units of the device (persistent grids)
     int inv_variant;    // fused inverse: 0 loads each plane at its start, 1 prefetches, 2 and 3 as 0 and 1
                         // with running top-t lists over contiguous planes (MDK_INV_VARIANT); at
-                        // nf = 256: 0 two passes, 1-3 the half-plane inverse
+                        // nf = 256: 0 two passes, 1 the half-plane inverse with 512 threads
+                        // reading each plane once (the default), 2-3 with 1024 reading it twice
     bool spectrum_only = false;  // stop after the fold: cc holds Q[b][kx][ky][kz]
     int fwd_variant = 0;         // forward passes: 0 fused z + y where it fits, the next plane prefetched (nf = 256: over half planes), 1 separate, 2 fused without the prefetch (MDK_FWD_VARIANT)
 };

@@ -25,7 +25,7 @@ constexpr Defaults kDefaults[] = {
     {88, 6, 3, 0}, {90, 6, 2, 0}, {96, 7, 3, 0}, {98, 7, 3, 0}, {100, 6, 3, 0}, {108, 5, 3, 0},
     {110, 7, 3, 0}, {112, 6, 3, 0}, {120, 5, 3, 0}, {126, 6, 3, 0}, {128, 5, 2, 0}, {132, 7, 3, 0},
     {140, 6, 3, 0}, {144, 6, 3, 0}, {150, 5, 2, 0}, {154, 6, 2, 0}, {160, 7, 2, 0}, {162, 5, 2, 0},
-    {168, 7, 2, 0}, {176, 5, 2, 0}, {180, 6, 2, 2}, {192, 6, 2, 2}, {256, 6, 2, 0},
+    {168, 7, 2, 0}, {176, 5, 2, 0}, {180, 6, 2, 2}, {192, 6, 2, 2}, {256, 6, 1, 0},
 };
 template <int N> constexpr int further_inv_default() { return N <= 128 && inv_prefetch_fits<N>() ? 3 : 2; }
 }  // namespace

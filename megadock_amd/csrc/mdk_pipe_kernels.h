@@ -1005,23 +1005,31 @@ template <int N> constexpr bool half_inverse_fits() {
            1024 % N1 == 0 && 1024 % N2 == 0 && M % (1024 / N1) == 0 && M * N2 <= 1024;
 }
 
-template <int N, int TT>
-__global__ void __launch_bounds__(1024)
+// NTH threads a workgroup.  With 1024 the odd half reads the plane again.  With 512
+// (KEEP: two waves per SIMD, so twice the registers) every thread takes two tasks of each
+// step, and the plane is read once: the odd rows' folded y inputs, (X[k] - X[k + M]) w^k,
+// are made from the same loads as the even rows' and wait in registers meanwhile.
+template <int N, int TT, int NTH = 1024>
+__global__ void __launch_bounds__(NTH)
     kp_inv_yz_half(const float2 *__restrict__ cc, Cand *__restrict__ cand, float *__restrict__ smap,
                    const float2 *__restrict__ twg, int Hp, int t, float nf3f, int nplanes) {
-    constexpr int NTH = 1024, NW = NTH / 64;
+    constexpr int NW = NTH / 64;
+    constexpr bool KEEP = NTH < 1024;
     constexpr int M = N / 2;
     constexpr int LSP = plane_stride(N);  // row stride of the half-plane (odd)
     constexpr int N1 = Geo<M>::N1, N2 = Geo<M>::N2;  // both transforms are M-point
     // the y transform runs in rounds of whole columns, one task per thread: step 1 over
     // CS1 = NTH / N2 columns a round, step 2 over CS2 = NTH / N1 columns a round (each round's
     // columns are read, then written back in place, so no thread holds two tasks' outputs);
-    // the z transform over RS = NTH / N1 rows a round, its outputs pushed round by round
+    // the z transform's step 1 in ZR1 rounds of NTH / N2 rows, all read before any is
+    // written, its step 2 over RS = NTH / N1 rows a round, its outputs pushed round by round
     constexpr int CS1 = NTH / N2, CS2 = NTH / N1, RS = NTH / N1;
     constexpr int HC = M;  // y columns: kz = 1 .. M - 1 and the packed pair kz = 0, M
     constexpr int R1 = (HC + CS1 - 1) / CS1, R2 = (HC + CS2 - 1) / CS2, RZ = M / RS;
+    constexpr int ZR1 = M * N2 / NTH;
     static_assert(half_inverse_fits<N>(), "half-plane inverse does not fit");
-    static_assert(NTH % N1 == 0 && NTH % N2 == 0 && M % RS == 0 && M * N2 <= NTH, "whole columns and rows a round");
+    static_assert(NTH % N1 == 0 && NTH % N2 == 0 && M % RS == 0 && (M * N2) % NTH == 0 && HC % CS1 == 0,
+                  "whole columns and rows a round");
     __shared__ float2 P[M * LSP];
     __shared__ float2 tw[N];
     __shared__ Cand wl[NW * TT];
@@ -1038,6 +1046,7 @@ __global__ void __launch_bounds__(1024)
     for (int p = p_first; p < p_end; p++) {  // uniform across the block
         const int b = p / N, x = p - b * N;
         const float2 *src = cc + (size_t)p * N * Hp;  // uniform base, 32-bit lane offsets
+        float2 keep[KEEP ? R1 : 1][KEEP ? N1 : 1];
         for (int h = 0; h < 2; h++) {  // uniform
             // opaque thread index: keeps the per-thread addresses out of the loops
             int tid = threadIdx.x;
@@ -1045,29 +1054,41 @@ __global__ void __launch_bounds__(1024)
             __syncthreads();  // P, tw and wl are free
             // ---- M-point y-IFFT of the H columns (line = kz, element = m), folded inputs;
             // thread (column yl = r CS1 + tid % CS1, index yu = tid / CS1) in round r
-#pragma unroll 1
-            for (int r = 0; r < R1; r++) {  // uniform
+            sfor<0, R1>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
                 const int yl = r * CS1 + tid % CS1, yu = tid / CS1;
-                if (yl < HC) {
-                    float2 v[N1];
+                float2 v[N1];
+                if (!KEEP || h == 0) {  // uniform
                     sfor<0, N1>([&](auto jc) {
                         constexpr int j1 = decltype(jc)::value;
                         const int k = yu + N2 * j1;
                         const float2 a = src[(unsigned)(k * Hp + yl)], c = src[(unsigned)((k + M) * Hp + yl)];
                         v[j1] = fold_half(a, c, h, tw[k]);
+                        if constexpr (KEEP) keep[KEEP ? r : 0][KEEP ? j1 : 0] = fold_half(a, c, 1, tw[k]);
                     });
                     if (yl == 0) {  // column 0 carries X(0) + i X(M): add i times the kz = M fold
                         asm volatile("" ::: "memory");  // after the main loads: no more of them in flight at once
                         sfor<0, N1>([&](auto jc) {
                             constexpr int j1 = decltype(jc)::value;
                             const int k = yu + N2 * j1;
-                            const float2 q = fold_half(src[(unsigned)(k * Hp + M)], src[(unsigned)((k + M) * Hp + M)], h, tw[k]);
+                            const float2 an = src[(unsigned)(k * Hp + M)], cn = src[(unsigned)((k + M) * Hp + M)];
+                            const float2 q = fold_half(an, cn, h, tw[k]);
                             v[j1] = make_float2(v[j1].x - q.y, v[j1].y + q.x);
+                            if constexpr (KEEP) {
+                                float2 &kk = keep[KEEP ? r : 0][KEEP ? j1 : 0];
+                                const float2 q1 = fold_half(an, cn, 1, tw[k]);
+                                kk = make_float2(kk.x - q1.y, kk.y + q1.x);
+                            }
                         });
                     }
-                    rf::step1_g<M, true>(v, P, tw, 2, LSP, 1, yl, yu);
+                } else {  // the odd rows' inputs, folded from the even rows' loads
+                    sfor<0, N1>([&](auto jc) {
+                        constexpr int j1 = decltype(jc)::value;
+                        v[j1] = keep[KEEP ? r : 0][KEEP ? j1 : 0];
+                    });
                 }
-            }
+                rf::step1_g<M, true>(v, P, tw, 2, LSP, 1, yl, yu);
+            });
             __syncthreads();
 #pragma unroll 1
             for (int r = 0; r < R2; r++) {  // uniform
@@ -1085,9 +1106,10 @@ __global__ void __launch_bounds__(1024)
             // ---- M-point inverse of each of the M rows with the complex-to-real pre-step:
             // Z[k] = (X[k] + conj(X[M-k])) + i w^k (X[k] - conj(X[M-k])), w^k = conj(tw[k])
             {
-                const int line = tid % M, u = tid / M;  // step 1: M rows x N2 tasks, one round
-                float2 v[N1];
-                if (u < N2)
+                float2 v[ZR1][N1];
+                sfor<0, ZR1>([&](auto rc) {  // step 1: M rows x N2 tasks, ZR1 rounds
+                    constexpr int r = decltype(rc)::value;
+                    const int q = r * NTH + tid, line = q % M, u = q / M;
                     sfor<0, N1>([&](auto jc) {
                         constexpr int j1 = decltype(jc)::value;
                         const int k = u + N2 * j1;
@@ -1099,10 +1121,15 @@ __global__ void __launch_bounds__(1024)
                         const float2 s1 = make_float2(a.x + c.x, a.y - c.y);
                         const float2 d1 = make_float2(a.x - c.x, a.y + c.y);
                         const float2 e1 = make_float2(w.x * d1.x + w.y * d1.y, w.x * d1.y - w.y * d1.x);
-                        v[j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
+                        v[r][j1] = make_float2(s1.x - e1.y, s1.y + e1.x);
                     });
+                });
                 __syncthreads();
-                if (u < N2) rf::step1_g<M, true>(v, P, tw, 2, 1, LSP, line, u);
+                sfor<0, ZR1>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    const int q = r * NTH + tid, line = q % M, u = q / M;
+                    rf::step1_g<M, true>(v[r], P, tw, 2, 1, LSP, line, u);
+                });
                 __syncthreads();
             }
             // step 2 and the top-t, RS rows a round: row `line` of the half is y = 2 line + h;
@@ -1158,8 +1185,11 @@ __global__ void __launch_bounds__(1024)
 
 template <int N, int TT> void launch_inv_half(const PipeArgs &a, hipStream_t s) {
     const int nplanes = a.B * N;
-    const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one 1024-thread workgroup per CU
-    kp_inv_yz_half<N, TT><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    const int grid = nplanes < a.cu_count ? nplanes : a.cu_count;  // one workgroup per CU
+    if (a.inv_variant == 1)  // 512 threads, the plane read once
+        kp_inv_yz_half<N, TT, 512><<<grid, 512, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
+    else
+        kp_inv_yz_half<N, TT><<<grid, 1024, 0, s>>>(a.cc, a.cand, a.smap, a.tw4, a.Hp, a.t, a.nf3f, nplanes);
 }
 
 // the fused inverse can prefetch the next plane (variants 1, 3) where a thread holds one

@@ -633,14 +633,15 @@ def test_graph_replays_match_eager(gpu, gcq):
 _NF256_REF = {}
 
 
-@pytest.mark.parametrize("inv,fwd", [("0", "1"), ("2", "0"), ("2", "1"), ("0", "0")])
+@pytest.mark.parametrize("inv,fwd", [("0", "1"), ("2", "0"), ("2", "1"), ("0", "0"), ("1", "0")])
 def test_nf256_inverse_forms_vs_oracle(gpu, synth, inv, fwd):
     """At nf = 256 a plane does not fit one workgroup's LDS.  The default fused forward pass
     takes each z-transformed plane as its even-kz and odd-kz halves and splits the y
     transform by ky parity (MDK_FWD_VARIANT 0), variant 1 runs the z and y passes apart; the
     default fused inverse takes each plane as its even and odd rows (a radix-2 split of the
-    y transform, MDK_INV_VARIANT 1-3), variant 0 runs the in-place y pass and the z pass
-    apart.  Each pairing against the oracle, per rotation, for t = 1 and t = 3 lists."""
+    y transform, MDK_INV_VARIANT 2-3; variant 1 with 512 threads reading the plane once),
+    variant 0 runs the in-place y pass and the z pass apart.  Each pairing against the
+    oracle, per rotation, for t = 1 and t = 3 lists."""
     rec, lig = synth
     ng, nf = 128, 256
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
