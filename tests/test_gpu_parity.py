@@ -241,22 +241,22 @@ def test_further_sizes_vs_fp64_and_oracle(gpu, synth, nf, t):
     ctx.close()
 
 
-@pytest.mark.parametrize("nf", [n for n in M.FURTHER_NF if n > 192])
-def test_sizes_above_192_vs_oracle(gpu, synth, nf):
+@pytest.mark.parametrize("nf,t", [(n, 2) for n in M.FURTHER_NF if n > 192] + [(n, 1) for n in (196, 250, 320)])
+def test_sizes_above_192_vs_oracle(gpu, synth, nf, t):
     """The table sizes above 192 (specialised with the passes that do not need a plane in
     LDS: forward z then y through HBM, the correlation pass, inverse y in place, then the
     z step over row groups, the last one partial where 16 does not divide nf): per-rotation
-    top-2 against the oracle on two rotations; at 200, 252 and 320 also the score map
-    against FP64."""
+    top-2 against the oracle on two rotations, and at three sizes the reference's default
+    t = 1 (raw score keys); at 200, 252 and 320 also the score map against FP64."""
     rec, lig = synth
     ng = nf // 2
     gr, gi = O.receptor_voxel(ng, 1.2, rec)
     rr, ri = O.receptor_fft(ng, nf, gr, gi)
-    ctx = _lig_ctx(lig, ng, t=2, max_batch=4)
+    ctx = _lig_ctx(lig, ng, t=t, max_batch=4)
     ctx.set_receptor(gr, gi)
     ang = O.load_rotations(3600)
     remb = O.embed(ng, nf, gr, gi)
-    if nf in (200, 252, 320):
+    if nf in (200, 252, 320) and t == 2:
         _say("size nf=%d: score map" % nf)
         a = 2345
         smap = ctx.score_map(M.rotation_coefs(ang[a:a + 1])[0]).astype(np.float64)
@@ -266,15 +266,15 @@ def test_sizes_above_192_vs_oracle(gpu, synth, nf):
         assert np.abs(smap - ref).max() <= 2e-6 * np.abs(ref).max()
         del smap, ref
     ids = np.array([11, 2000], dtype=np.int32)
-    _say("size nf=%d: dock %d rotations" % (nf, len(ids)))
+    _say("size nf=%d: dock %d rotations, t = %d" % (nf, len(ids), t))
     got = ctx.dock(M.rotation_coefs(ang[ids]), ids)
-    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, 2, threads=16)
-    for q in range(len(ids) * 2):
+    exp = O.dock(ng, nf, 1.2, lig, rr, ri, ang, ids, t, threads=16)
+    for q in range(len(ids) * t):
         g, r = got[q], exp[q]
         assert g["index"][0] == r["index"][0]
         assert abs(g["score"] - r["score"]) <= 1e-4 * max(abs(r["score"]), 1e-3), (q, g, r)
         if not np.array_equal(g["index"], r["index"]):
-            b = int(ids[q // 2])
+            b = int(ids[q // t])
             xyz = O.rotate_ligand(lig.xyz, lig.center, O.rotation_coef(ang[b]))
             lr, li = O.ligand_voxel(ng, 1.2, xyz, lig.radius, lig.charge)
             s64 = O.correlation_fp64(nf, remb, O.embed(ng, nf, lr, li)).ravel()
